@@ -1,0 +1,337 @@
+"""CPU oracle for the MPM substep -- TEST INFRASTRUCTURE ONLY.
+
+A ctypes front end over ``oracle/libmpm_oracle.so`` (built from
+``oracle/mpm_oracle.cpp`` by ``oracle/Makefile``).  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import
+this package; the product package ``molla_amd`` never does.
+
+Scenes are described by any object with the attributes of
+``molla_amd.MpmConfig`` (grid_size, cell_size, grid_origin, gravity_y, youngs,
+poisson, boundary_friction, model, floor_y, floor_friction, materials,
+grid_window_cells) so tests can hand the same configuration to the product and
+to the oracle.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libmpm_oracle.so")
+
+MAX_MATERIALS = 8
+MODEL_SHADER = 0
+MODEL_CPUDIFF = 1
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("rheology", C.c_float),
+        ("dp_alpha", C.c_float),
+        ("theta_c", C.c_float),
+        ("theta_s", C.c_float),
+        ("hardening", C.c_float),
+        ("velocity_damping", C.c_float),
+        ("affine_damping", C.c_float),
+        ("bulk", C.c_float),
+    ]
+
+
+class Scene(C.Structure):
+    _fields_ = [
+        ("grid_size", C.c_uint32),
+        ("cell_size", C.c_float),
+        ("origin", C.c_float * 3),
+        ("window", C.c_int32 * 3),
+        ("gravity_y", C.c_float),
+        ("youngs", C.c_float),
+        ("poisson", C.c_float),
+        ("boundary_friction", C.c_float),
+        ("model", C.c_uint32),
+        ("floor_y", C.c_float),
+        ("floor_friction", C.c_float),
+        ("n_materials", C.c_uint32),
+        ("materials", Material * MAX_MATERIALS),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle library (gcc); returns its path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _p(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def scene_from(cfg) -> Scene:
+    """Build an oracle scene from an MpmConfig-like object."""
+    s = Scene()
+    s.grid_size = int(cfg.grid_size)
+    s.cell_size = float(cfg.cell_size)
+    for a in range(3):
+        s.origin[a] = float(cfg.grid_origin[a])
+        s.window[a] = int(getattr(cfg, "grid_window_cells", (0, 0, 0))[a])
+    s.gravity_y = float(cfg.gravity_y)
+    s.youngs = float(cfg.youngs)
+    s.poisson = float(cfg.poisson)
+    s.boundary_friction = float(cfg.boundary_friction)
+    s.model = int(getattr(cfg, "model", MODEL_SHADER))
+    s.floor_y = float(getattr(cfg, "floor_y", 0.0))
+    s.floor_friction = float(getattr(cfg, "floor_friction", 0.0))
+    rows = list(cfg.material_rows())
+    s.n_materials = len(rows)
+    for i, r in enumerate(rows):
+        m = s.materials[i]
+        (m.rheology, m.dp_alpha, m.theta_c, m.theta_s, m.hardening, m.velocity_damping,
+         m.affine_damping, m.bulk) = [float(t) for t in r]
+    return s
+
+
+# ---------------------------------------------------------------- ordering
+def tile_keys(cfg, x: np.ndarray, capacity: int | None = None):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    n = x.shape[0]
+    cap = n if capacity is None else int(capacity)
+    keys = np.zeros(cap, np.uint32)
+    ids = np.zeros(cap, np.uint32)
+    s = scene_from(cfg)
+    lib().orc_tile_keys(C.byref(s), C.c_uint32(n), C.c_uint32(cap), _p(x, C.c_float),
+                        _p(keys, C.c_uint32), _p(ids, C.c_uint32))
+    return keys, ids
+
+
+def sort_pairs(keys: np.ndarray, ids: np.ndarray):
+    k = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    i = np.ascontiguousarray(ids, dtype=np.uint32).copy()
+    lib().orc_sort_pairs(C.c_uint32(k.size), _p(k, C.c_uint32), _p(i, C.c_uint32))
+    return k, i
+
+
+def sorted_tile_keys(cfg, x, capacity=None):
+    k, i = tile_keys(cfg, x, capacity)
+    return sort_pairs(k, i)
+
+
+def base_cells(cfg, x):
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((x.shape[0], 3), np.int32)
+    s = scene_from(cfg)
+    lib().orc_base_cells(C.byref(s), C.c_uint32(x.shape[0]), _p(x, C.c_float), _p(out, C.c_int32))
+    return out
+
+
+# ---------------------------------------------------------------- matrices
+def _m(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(9)
+
+
+def svd_shader(F, dtype=np.float64):
+    if dtype == np.float32:
+        f = np.ascontiguousarray(F, dtype=np.float32).reshape(9)
+        U = np.zeros(9, np.float32); s = np.zeros(3, np.float32); V = np.zeros(9, np.float32)
+        lib().orc_svd3_shader_f32(_p(f, C.c_float), _p(U, C.c_float), _p(s, C.c_float), _p(V, C.c_float))
+        return U.reshape(3, 3), s, V.reshape(3, 3)
+    f = _m(F)
+    U = np.zeros(9); s = np.zeros(3); V = np.zeros(9)
+    lib().orc_svd3_shader_f64(_p(f, C.c_double), _p(U, C.c_double), _p(s, C.c_double), _p(V, C.c_double))
+    return U.reshape(3, 3), s, V.reshape(3, 3)
+
+
+def svd_cpu(F):
+    f = _m(F)
+    U = np.zeros(9); s = np.zeros(3); V = np.zeros(9)
+    lib().orc_svd3_cpu(_p(f, C.c_double), _p(U, C.c_double), _p(s, C.c_double), _p(V, C.c_double))
+    return U.reshape(3, 3), s, V.reshape(3, 3)
+
+
+def adj_svd(U, s, V, aU, aS, aV):
+    out = np.zeros(9)
+    lib().orc_adj_svd3(_p(_m(U), C.c_double), _p(np.ascontiguousarray(s, np.float64), C.c_double),
+                       _p(_m(V), C.c_double), _p(_m(aU), C.c_double),
+                       _p(np.ascontiguousarray(aS, np.float64), C.c_double), _p(_m(aV), C.c_double),
+                       _p(out, C.c_double))
+    return out.reshape(3, 3)
+
+
+def pk1_fixed_corotated(F, mu, lam):
+    out = np.zeros(9)
+    lib().orc_pk1_fixed_corotated(_p(_m(F), C.c_double), C.c_double(mu), C.c_double(lam), _p(out, C.c_double))
+    return out.reshape(3, 3)
+
+
+def pk1_corotated_svk(F, mu, lam):
+    out = np.zeros(9)
+    lib().orc_pk1_corotated_svk(_p(_m(F), C.c_double), C.c_double(mu), C.c_double(lam), _p(out, C.c_double))
+    return out.reshape(3, 3)
+
+
+def pk1_corotated_svk_adjoint(F, mu, lam, adjP):
+    out = np.zeros(9)
+    amu = C.c_double(0.0)
+    alam = C.c_double(0.0)
+    lib().orc_pk1_corotated_svk_adjoint(_p(_m(F), C.c_double), C.c_double(mu), C.c_double(lam),
+                                        _p(_m(adjP), C.c_double), _p(out, C.c_double),
+                                        C.byref(amu), C.byref(alam))
+    return out.reshape(3, 3), amu.value, alam.value
+
+
+def quad_weights(fx):
+    w = np.zeros(3)
+    lib().orc_quad_weights(C.c_double(fx), _p(w, C.c_double))
+    return w
+
+
+def quad_weights_dfx(fx):
+    w = np.zeros(3)
+    lib().orc_quad_weights_dfx(C.c_double(fx), _p(w, C.c_double))
+    return w
+
+
+def plastic_project(cfg, material_row, sigma, jp):
+    s = scene_from(cfg)
+    m = Material(*[float(t) for t in material_row])
+    sv = np.ascontiguousarray(sigma, np.float64)
+    out = np.zeros(3)
+    jo = C.c_double(0.0)
+    lib().orc_plastic_project(C.byref(s), C.byref(m), _p(sv, C.c_double), C.c_double(jp),
+                              _p(out, C.c_double), C.byref(jo))
+    return out, jo.value
+
+
+# ---------------------------------------------------------------- stepping
+class State:
+    """Particle state arrays (row-major matrices) in a chosen precision."""
+
+    def __init__(self, x, v=None, C_=None, F=None, jp=None, mass=None, vol=None, mat=None,
+                 E=None, dtype=np.float64):
+        x = np.asarray(x, dtype=dtype).reshape(-1, 3)
+        n = x.shape[0]
+        self.dtype = dtype
+        self.x = np.ascontiguousarray(x.copy())
+        self.v = np.ascontiguousarray(np.zeros((n, 3), dtype) if v is None else np.asarray(v, dtype).reshape(n, 3).copy())
+        self.C = np.ascontiguousarray(np.zeros((n, 3, 3), dtype) if C_ is None else np.asarray(C_, dtype).reshape(n, 3, 3).copy())
+        if F is None:
+            F = np.broadcast_to(np.eye(3, dtype=dtype), (n, 3, 3))
+        self.F = np.ascontiguousarray(np.asarray(F, dtype).reshape(n, 3, 3).copy())
+        self.jp = np.ascontiguousarray(np.zeros(n, dtype) if jp is None else np.asarray(jp, dtype).copy())
+        self.mass = np.array(mass, dtype).reshape(n) if mass is not None else np.zeros(n, dtype)
+        self.vol = np.array(vol, dtype).reshape(n) if vol is not None else np.zeros(n, dtype)
+        self.mat = None if mat is None else np.array(mat, np.uint32).reshape(n)
+        self.E = None if E is None else np.array(E, dtype).reshape(n)
+        self.status = np.zeros(2, np.uint32)
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+    def copy(self):
+        s = State(self.x, self.v, self.C, self.F, self.jp, self.mass, self.vol, self.mat, self.E, self.dtype)
+        s.status = self.status.copy()
+        return s
+
+
+def step(cfg, st: State, dt: float, substeps: int, want_grid=False):
+    """Run `substeps` substeps in place; returns (grid_m, grid_p) of the last P2G if asked."""
+    s = scene_from(cfg)
+    G = int(cfg.grid_size)
+    ct = C.c_double if st.dtype == np.float64 else C.c_float
+    gm = np.zeros(G ** 3, st.dtype) if want_grid else None
+    gp = np.zeros((G ** 3, 3), st.dtype) if want_grid else None
+    fn = lib().orc_step_f64 if st.dtype == np.float64 else lib().orc_step_f32
+    rc = fn(C.byref(s), C.c_uint32(st.n), _p(st.x, ct), _p(st.v, ct), _p(st.C, ct), _p(st.F, ct),
+            _p(st.jp, ct), _p(st.mass, ct), _p(st.vol, ct), _p(st.mat, C.c_uint32), _p(st.E, ct),
+            ct(dt), C.c_uint32(substeps), _p(st.status, C.c_uint32), _p(gm, ct), _p(gp, ct))
+    if rc != 0:
+        raise RuntimeError(f"oracle step failed rc={rc}")
+    return gm, gp
+
+
+def p2g(cfg, st: State, dt: float):
+    s = scene_from(cfg)
+    G = int(cfg.grid_size)
+    gm = np.zeros(G ** 3)
+    gp = np.zeros((G ** 3, 3))
+    d = st if st.dtype == np.float64 else st.copy()
+    if d.dtype != np.float64:
+        d = State(st.x, st.v, st.C, st.F, st.jp, st.mass, st.vol, st.mat, st.E, np.float64)
+    rc = lib().orc_p2g_f64(C.byref(s), C.c_uint32(d.n), _p(d.x, C.c_double), _p(d.v, C.c_double),
+                           _p(d.C, C.c_double), _p(d.F, C.c_double), _p(d.jp, C.c_double),
+                           _p(d.mass, C.c_double), _p(d.vol, C.c_double), _p(d.mat, C.c_uint32),
+                           _p(d.E, C.c_double), C.c_double(dt), _p(gm, C.c_double), _p(gp, C.c_double))
+    if rc != 0:
+        raise RuntimeError(f"oracle p2g failed rc={rc}")
+    return gm, gp
+
+
+def p2g_gather_f32(cfg, st: State, dt: float):
+    s = scene_from(cfg)
+    G = int(cfg.grid_size)
+    d = st if st.dtype == np.float32 else State(st.x, st.v, st.C, st.F, st.jp, st.mass, st.vol,
+                                                st.mat, st.E, np.float32)
+    gm = np.zeros(G ** 3, np.float32)
+    gp = np.zeros((G ** 3, 3), np.float32)
+    rc = lib().orc_p2g_gather_f32(C.byref(s), C.c_uint32(d.n), _p(d.x, C.c_float), _p(d.v, C.c_float),
+                                  _p(d.C, C.c_float), _p(d.F, C.c_float), _p(d.jp, C.c_float),
+                                  _p(d.mass, C.c_float), _p(d.vol, C.c_float), _p(d.mat, C.c_uint32),
+                                  _p(d.E, C.c_float), C.c_float(dt), _p(gm, C.c_float), _p(gp, C.c_float))
+    if rc != 0:
+        raise RuntimeError(f"oracle gather p2g failed rc={rc}")
+    return gm, gp
+
+
+def diff_forward(cfg, st: State, dt: float, steps: int, record=True):
+    """CPU differentiable forward (f64, CPUDIFF model). Returns the tape (or None)."""
+    s = scene_from(cfg)
+    s.model = MODEL_CPUDIFF
+    tape = np.zeros(steps * st.n * 24) if record else None
+    rc = lib().orc_diff_forward(C.byref(s), C.c_uint32(st.n), _p(st.x, C.c_double), _p(st.v, C.c_double),
+                                _p(st.C, C.c_double), _p(st.F, C.c_double), _p(st.mass, C.c_double),
+                                _p(st.vol, C.c_double), _p(st.E, C.c_double), C.c_double(dt),
+                                C.c_uint32(steps), _p(tape, C.c_double))
+    if rc != 0:
+        raise RuntimeError("oracle diff forward failed")
+    return tape
+
+
+def diff_backward(cfg, st0: State, dt: float, steps: int, tape, gx, gv=None, gC=None, gF=None,
+                  per_particle_E=True):
+    """Reverse pass; returns dict of gradients w.r.t. the initial state and E."""
+    s = scene_from(cfg)
+    s.model = MODEL_CPUDIFF
+    n = st0.n
+    gx = np.ascontiguousarray(np.asarray(gx, np.float64).reshape(n, 3).copy())
+    gv = np.ascontiguousarray(np.zeros((n, 3)) if gv is None else np.asarray(gv, np.float64).reshape(n, 3).copy())
+    gC = np.ascontiguousarray(np.zeros((n, 3, 3)) if gC is None else np.asarray(gC, np.float64).reshape(n, 3, 3).copy())
+    gF = np.ascontiguousarray(np.zeros((n, 3, 3)) if gF is None else np.asarray(gF, np.float64).reshape(n, 3, 3).copy())
+    gE = np.zeros(n) if per_particle_E else None
+    gml = np.zeros(2)
+    rc = lib().orc_diff_backward(C.byref(s), C.c_uint32(n), _p(st0.mass, C.c_double), _p(st0.vol, C.c_double),
+                                 _p(st0.E, C.c_double), C.c_double(dt), C.c_uint32(steps),
+                                 _p(tape, C.c_double), _p(gx, C.c_double), _p(gv, C.c_double),
+                                 _p(gC, C.c_double), _p(gF, C.c_double), _p(gE, C.c_double),
+                                 _p(gml, C.c_double))
+    if rc != 0:
+        raise RuntimeError("oracle diff backward failed")
+    nu = float(cfg.poisson)
+    g_youngs = gml[0] / (2 * (1 + nu)) + gml[1] * nu / ((1 + nu) * (1 - 2 * nu))
+    return {"x": gx, "v": gv, "C": gC, "F": gF, "E": gE, "mu": gml[0], "lambda": gml[1],
+            "youngs": g_youngs}
