@@ -1,0 +1,202 @@
+/*
+ * molla_mpm.h -- C ABI of the MI355X-native MPM substep (libmolla_mpm.so).
+ *
+ * Drop-in boundary for molla's `MpmSimulation` stepping API
+ * (crates/molla-solvers/src/mpm.rs).  Every entry point names the reference
+ * item it replaces.  Plain pointers and sizes only; all host arrays are
+ * little-endian f32/u32, particles in their original (caller) order:
+ *   positions / velocities : n * 3 floats  ([f32; 3] per particle)
+ *   C, F                   : n * 9 floats  (row-major 3x3 per particle)
+ *   grid mass              : G^3 floats, momentum 3 * G^3, cell = x + G(y + G z)
+ *
+ * Errors: every call returns an mpm_status; the message of the last failure
+ * on a handle is available from mpm_last_error() (reference:
+ * molla_core::Error::{Solver, Gpu}, crates/molla-core/src/error.rs:10-39).
+ * Device faults are latched, not returned, and are read with
+ * mpm_read_status() (reference: read_active_grid_status_with, mpm.rs:1418).
+ *
+ * Threading: a handle is externally synchronised (one host thread at a time,
+ * like the reference's `&mut self`); distinct handles are independent.  All
+ * work is ordered on the handle's HIP stream; reads block until it drains.
+ */
+#ifndef MOLLA_MPM_H
+#define MOLLA_MPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_ABI_VERSION 1
+#define MPM_MAX_MATERIALS 8 /* MAX_MPM_MATERIALS, mpm.rs:661 */
+
+typedef enum {
+    MPM_OK = 0,
+    MPM_ERR_INVALID = 1, /* Error::Solver: validation failure          */
+    MPM_ERR_DEVICE = 2,  /* Error::Gpu: HIP runtime / allocation error */
+    MPM_ERR_FAULT = 3    /* a latched device fault blocks the request  */
+} mpm_status;
+
+/* Semantics: the GPU step of mpm.wgsl, or the differentiable
+ * CPU step of molla-autograd/src/mpm_cpu_diff.rs. */
+typedef enum { MPM_MODEL_SHADER = 0, MPM_MODEL_CPUDIFF = 1 } mpm_model;
+
+/* Rheology ids of the material table (material_row, mpm.rs:666-686);
+ * MPM_RHEO_FLUID is this library's weakly-compressible fluid. */
+typedef enum {
+    MPM_RHEO_SAND = 0,
+    MPM_RHEO_SNOW = 1,
+    MPM_RHEO_VISCOUS = 2,
+    MPM_RHEO_FLUID = 3
+} mpm_rheology;
+
+/* One material-table row, in the reference's packing order
+ * (p0 = rheology, dp_alpha, theta_c, theta_s; p1 = hardening,
+ * velocity_damping, affine_damping, <bulk>). */
+typedef struct {
+    float rheology;
+    float dp_alpha;
+    float theta_c;
+    float theta_s;
+    float hardening;
+    float velocity_damping;
+    float affine_damping;
+    float bulk;
+} mpm_material;
+
+/* Mirrors MpmConfig (mpm.rs:394-433) for one scene. */
+typedef struct {
+    uint32_t grid_size;
+    float cell_size;
+    float grid_origin[3];
+    int32_t grid_window_cells[3]; /* MpmSimulation::grid_window_cells, mpm.rs:400-402 */
+    float gravity_y;
+    float youngs;
+    float poisson;
+    float boundary_friction;
+    float particle_mass;
+    float particle_volume;
+    uint32_t max_particles;
+    uint32_t model;   /* mpm_model */
+    float floor_y;    /* MPM_MODEL_CPUDIFF: MpmCpuConfig::floor_y        */
+    float floor_friction; /* MPM_MODEL_CPUDIFF: 1 = sticky (mpm_cpu_diff.rs:96-99) */
+    uint32_t n_materials; /* rows used in `materials` (>= 1)             */
+    mpm_material materials[MPM_MAX_MATERIALS];
+} mpm_scene_config;
+
+typedef struct mpm_sim mpm_sim;
+
+/* Stepping options (set_p2g_mode, mpm.rs:1642; set_particle_tile_ordering,
+ * mpm_particle_order.rs:244; set_deterministic_p2g,
+ * mpm_deterministic_p2g.rs:550). */
+typedef enum {
+    MPM_P2G_FUSED = 0,         /* fused G2P->P2G kernel, LDS-staged (default)      */
+    MPM_P2G_GLOBAL_ATOMIC = 1, /* unfused, one global atomic per node (legacy p2g) */
+    MPM_P2G_DETERMINISTIC = 2  /* sorted gather-order P2G, bitwise repeatable      */
+} mpm_p2g_mode;
+
+typedef struct {
+    uint32_t fault_bits;        /* 1 overflow, 2 numerical (mpm.wgsl:175-231) */
+    uint32_t completed_gathers; /* G2P passes run (g_kdiag metadata word 3)   */
+    uint32_t active_tiles;      /* 4^3 tiles touched by the last P2G          */
+    uint32_t sorts;             /* particle re-sorts performed                */
+} mpm_status_info;
+
+/* ---- lifetime (MpmSimulation::new / new_with_device, mpm.rs:834-853) ---- */
+int mpm_create(const mpm_scene_config* cfg, mpm_sim** out);
+/* Batched: `n_scenes` independent scenes in one context (no reference
+ * counterpart: the reference runs one MpmSimulation per scene). */
+int mpm_create_batch(const mpm_scene_config* cfgs, uint32_t n_scenes, mpm_sim** out);
+int mpm_destroy(mpm_sim* sim);
+const char* mpm_last_error(const mpm_sim* sim);
+int mpm_abi_version(void);
+
+/* ---- particles (set_particles / set_particles_with_velocity, mpm.rs:1451-1519) ----
+ * F = I, C = 0, jp = 0, mass/volume from the config; `velocities` may be NULL. */
+int mpm_set_particles(mpm_sim* sim, const float* positions, const float* velocities, uint32_t n);
+/* Batched variant: counts[s] particles per scene, concatenated scene-major. */
+int mpm_set_particles_batch(mpm_sim* sim, const uint32_t* counts, const float* positions,
+                            const float* velocities);
+
+/* Per-particle fields (original order).  Field ids: */
+typedef enum {
+    MPM_FIELD_POSITION = 0, /* 3 */
+    MPM_FIELD_VELOCITY = 1, /* 3 */
+    MPM_FIELD_C = 2,        /* 9 */
+    MPM_FIELD_F = 3,        /* 9 */
+    MPM_FIELD_JP = 4,       /* 1  p_jp                                  */
+    MPM_FIELD_MASS = 5,     /* 1  p_mass                                */
+    MPM_FIELD_VOLUME = 6,   /* 1  p_volume                              */
+    MPM_FIELD_YOUNGS = 7,   /* 1  per-particle E (extension)            */
+    MPM_FIELD_MATERIAL = 8  /* 1  u32 row of the material table, p_material */
+} mpm_field;
+int mpm_write_field(mpm_sim* sim, int field, const void* data, uint32_t n);
+int mpm_read_field(mpm_sim* sim, int field, void* out, uint32_t n);
+/* read_positions_with / read_velocities_with (mpm.rs:2718-2790). */
+int mpm_read_positions(mpm_sim* sim, float* out, uint32_t n);
+int mpm_read_velocities(mpm_sim* sim, float* out, uint32_t n);
+
+/* ---- materials (set_materials / set_rheology, mpm.rs:1570-1635) ---- */
+int mpm_set_materials(mpm_sim* sim, uint32_t scene, const mpm_material* rows, uint32_t n_rows);
+int mpm_set_scene_params(mpm_sim* sim, uint32_t scene, const mpm_scene_config* cfg);
+
+/* ---- modes ---- */
+int mpm_set_p2g_mode(mpm_sim* sim, int mode);
+/* Re-sort particles by 4^3 tile every `every` substeps (0 = only when
+ * stepping starts).  Sorting never changes results, only locality. */
+int mpm_set_sort_interval(mpm_sim* sim, uint32_t every);
+/* Store v and C at every substep (1, default, reference semantics) or only
+ * at the last substep of a step_n call (0). */
+int mpm_set_write_vc(mpm_sim* sim, int every_substep);
+/* Launch tuning: blocks for the particle pass (0 = one per chunk). */
+int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks);
+
+/* ---- stepping (step / step_n, mpm.rs:2232-2272) ---- */
+int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps);
+/* Capture step_n(dt, substeps) into a hipGraph once and replay it
+ * (PreparedMpmStep / MpmCommandBatch, mpm_command_batch.rs:10-235). */
+int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps);
+int mpm_synchronize(mpm_sim* sim);
+
+/* ---- grid / status / ordering ---- */
+/* read_grid_mass_momentum (mpm.rs:2792-2906): the last substep's P2G. */
+int mpm_read_grid_mass_momentum(mpm_sim* sim, uint32_t scene, float* mass, float* momentum);
+int mpm_read_status(mpm_sim* sim, mpm_status_info* out);
+int mpm_reset_status(mpm_sim* sim); /* reset_active_grid, mpm.rs:1388 */
+/* Tile keys and stable sorted ids of the current positions, exactly as the
+ * ParticleOrder::encode path (mpm_particle_order.rs:100-146): keys of one
+ * scene, padded to `capacity` with 0xffffffff / identity ids. */
+int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_t* keys,
+                        uint32_t* ids);
+
+/* ---- profiling (step_n_profiled_with / read_step_profile, mpm_profile.rs:649-666) ---- */
+typedef enum {
+    MPM_STAGE_SORT = 0,
+    MPM_STAGE_P2G = 1,
+    MPM_STAGE_GRID = 2,
+    MPM_STAGE_G2P2G = 3,
+    MPM_STAGE_G2P = 4,
+    MPM_STAGE_COUNT = 5
+} mpm_stage;
+int mpm_set_profiling(mpm_sim* sim, int enabled);
+/* Accumulated milliseconds and launch counts per stage since the last reset. */
+int mpm_read_profile(mpm_sim* sim, float* ms, uint32_t* launches);
+int mpm_reset_profile(mpm_sim* sim);
+
+/* ---- differentiable path (mpm_cpu_diff.rs forward_with_tape :397-418 and
+ *      mpm_adjoint.rs backward :121-137), MPM_MODEL_CPUDIFF only ----
+ * Forward `steps` substeps keeping a checkpoint of the full particle state
+ * every `checkpoint_every` substeps. */
+int mpm_diff_forward(mpm_sim* sim, float dt, uint32_t steps, uint32_t checkpoint_every);
+/* Reverse pass.  grad_x_final: n*3 dJ/dx_T (required); grad_v_final may be
+ * NULL.  Outputs (each may be NULL): dJ/dx_0, dJ/dv_0 (n*3), dJ/dE_p (n),
+ * and the global (dJ/dmu, dJ/dlambda) accumulators of StateAdjoint
+ * (mpm_adjoint.rs:76-113). */
+int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad_v_final,
+                      float* grad_x0, float* grad_v0, float* grad_E, double* grad_mu_lambda);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOLLA_MPM_H */

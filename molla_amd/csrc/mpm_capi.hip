@@ -1,0 +1,889 @@
+// mpm_capi.hip -- host runtime behind include/molla_mpm.h.
+//
+// One context (`mpm_sim`) owns a HIP stream and every device buffer of one or
+// more scenes (a single scene is a batch of one).  Stepping never allocates
+// and never copies to or from the host; the only host<->device traffic is the
+// explicit set/read calls, mirroring the reference's persistent-buffer design
+// (book/src/compute/persistent.md; mpm.rs:1030-1063).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/molla_mpm.h"
+#include "mpm_internal.hpp"
+
+using mpm::Chunk;
+using mpm::SceneDev;
+
+namespace {
+
+constexpr uint32_t CHUNK = 256;
+
+struct GraphKey {
+    uint32_t dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks;
+    bool operator<(const GraphKey& o) const {
+        return std::tie(dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks) <
+               std::tie(o.dt_bits, o.n, o.mode, o.ps_cur, o.list_cur, o.dirty, o.sort_start, o.write_vc,
+                        o.pblocks, o.gblocks);
+    }
+};
+
+struct ProfRec {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct mpm_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<mpm_scene_config> cfg;
+    std::vector<SceneDev> sh;
+    uint32_t n_scenes = 0;
+    uint32_t total_tiles = 0;
+    uint32_t cap = 0;
+    uint32_t n = 0;
+    std::vector<uint32_t> counts, pbase;
+    // device buffers
+    SceneDev* d_scenes = nullptr;
+    float* d_mats = nullptr;
+    float* ps[2] = {nullptr, nullptr};
+    int ps_cur = 0;
+    float4* gmp = nullptr;
+    float4* gv = nullptr;
+    uint32_t* tile_flag = nullptr;
+    uint32_t* tile_scene = nullptr;
+    uint32_t* list[2] = {nullptr, nullptr};
+    uint32_t* status = nullptr;  // ST_WORDS u32
+    uint32_t* slot_of_id = nullptr;
+    uint32_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    uint32_t* sort_scratch = nullptr;
+    Chunk* d_chunks = nullptr;
+    uint32_t n_chunks = 0;
+    uint32_t* tile_begin = nullptr;
+    uint32_t* tile_end = nullptr;
+    float* affine = nullptr;
+    float* staging = nullptr;  // cap * 9 floats
+    int list_cur = 0;
+    bool grid_dirty = false;
+    bool sorted = false;
+    uint32_t since_sort = 0;
+    uint32_t sorts = 0;
+    int has_E = 0;
+    // options
+    int p2g_mode = MPM_P2G_FUSED;
+    uint32_t sort_every = 16;
+    int write_vc = 1;
+    uint32_t pblocks = 0;
+    uint32_t gblocks = 0;
+    // profiling
+    bool profiling = false;
+    std::vector<ProfRec> prof_pending;
+    double prof_ms[MPM_STAGE_COUNT] = {0};
+    uint32_t prof_n[MPM_STAGE_COUNT] = {0};
+    // graphs
+    std::map<GraphKey, hipGraphExec_t> graphs;
+    bool capturing = false;
+    bool dry = false;  // walk the schedule's host bookkeeping without launching
+    // differentiable path
+    std::vector<float*> ckpt;  // checkpoints of the 24 state planes
+    uint32_t diff_steps = 0, diff_every = 0;
+    float diff_dt = 0.f;
+    std::string err;
+};
+
+namespace {
+
+int fail(mpm_sim* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    return code;
+}
+
+int hip_fail(mpm_sim* s, hipError_t e, const char* what) {
+    return fail(s, MPM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(s, expr)                               \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return hip_fail(s, _e, #expr); \
+    } while (0)
+
+uint32_t ceil_log2(uint32_t v) {
+    uint32_t b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
+
+SceneDev make_scene(const mpm_scene_config& c, uint32_t tile_base, uint32_t mat_base) {
+    SceneDev d{};
+    for (int k = 0; k < 3; ++k) {
+        d.origin[k] = c.grid_origin[k];
+        d.window[k] = c.grid_window_cells[k];
+    }
+    d.dx = c.cell_size;
+    d.inv_dx = 1.0f / c.cell_size;
+    d.gravity_y = c.gravity_y;
+    d.youngs = c.youngs;
+    d.poisson = c.poisson;
+    d.boundary_friction = c.boundary_friction;
+    d.floor_y = c.floor_y;
+    d.floor_friction = c.floor_friction;
+    d.mu = c.youngs / (2.0f * (1.0f + c.poisson));
+    d.lam = c.youngs * c.poisson / ((1.0f + c.poisson) * (1.0f - 2.0f * c.poisson));
+    d.G = c.grid_size;
+    d.W = (c.grid_size + 3u) / 4u;
+    d.tile_base = tile_base;
+    d.model = c.model;
+    d.mat_base = mat_base;
+    d.n_mat = c.n_materials;
+    return d;
+}
+
+std::string validate(const mpm_scene_config& c) {
+    if (c.grid_size < 5 || c.grid_size > 4096) return "grid_size must be in [5, 4096]";
+    if (!(c.cell_size > 0.0f) || !std::isfinite(c.cell_size)) return "cell_size must be finite and > 0";
+    if (c.n_materials < 1 || c.n_materials > MPM_MAX_MATERIALS) return "n_materials must be in [1, 8]";
+    if (c.model > 1) return "unknown model";
+    if (!(c.poisson > -1.0f && c.poisson < 0.5f)) return "poisson must be in (-1, 0.5)";
+    if (!std::isfinite(c.youngs) || !std::isfinite(c.gravity_y)) return "non-finite material/gravity";
+    return "";
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+void release(mpm_sim* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& kv : s->graphs) hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+    for (auto& r : s->prof_pending) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    void* bufs[] = {s->d_scenes, s->d_mats, s->ps[0], s->ps[1], s->gmp, s->gv, s->tile_flag, s->tile_scene,
+                    s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
+                    s->d_chunks, s->tile_begin, s->tile_end, s->affine, s->staging};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (float* c : s->ckpt) hipFree(c);
+    if (s->stream) hipStreamDestroy(s->stream);
+}
+
+mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
+    mpm::StepArgs a{};
+    a.scenes = s->d_scenes;
+    a.mats = s->d_mats;
+    a.ps = s->ps[s->ps_cur];
+    a.cap = s->cap;
+    a.gmp = s->gmp;
+    a.gv = s->gv;
+    a.tile_flag = s->tile_flag;
+    a.list_out = s->list[list_out];
+    a.count_out = s->status + mpm::ST_COUNT0 + list_out;
+    a.status = s->status;
+    a.chunks = s->d_chunks;
+    a.n_chunks = s->n_chunks;
+    a.dt = dt;
+    a.use_lds = s->p2g_mode == MPM_P2G_FUSED ? 1 : 0;
+    a.write_vc = s->write_vc;
+    a.has_E = s->has_E;
+    return a;
+}
+
+// ---- profiling helpers ----
+struct Stage {
+    mpm_sim* s;
+    ProfRec r{};
+    Stage(mpm_sim* s_, int stage) : s(s_) {
+        if (!s->profiling || s->capturing) return;
+        r.stage = stage;
+        hipEventCreate(&r.a);
+        hipEventCreate(&r.b);
+        hipEventRecord(r.a, s->stream);
+    }
+    ~Stage() {
+        if (!s->profiling || s->capturing) return;
+        hipEventRecord(r.b, s->stream);
+        s->prof_pending.push_back(r);
+    }
+};
+
+void flush_profile(mpm_sim* s) {
+    if (s->prof_pending.empty()) return;
+    hipStreamSynchronize(s->stream);
+    for (auto& r : s->prof_pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        s->prof_ms[r.stage] += ms;
+        s->prof_n[r.stage] += 1;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    s->prof_pending.clear();
+}
+
+int grid_blocks(mpm_sim* s) {
+    if (s->gblocks) return int(s->gblocks);
+    return 2048;
+}
+int particle_blocks(mpm_sim* s) {
+    if (s->pblocks) return int(std::min(s->pblocks, s->n_chunks));
+    return int(s->n_chunks);
+}
+
+// The dense tile grid is allocated on first use, so contexts that only order
+// particles (or read keys) never pay for it.
+int ensure_grid(mpm_sim* s) {
+    if (s->gmp) return MPM_OK;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    HIP_TRY(s, dalloc(&s->gmp, cells));
+    HIP_TRY(s, dalloc(&s->gv, cells));
+    HIP_TRY(s, dalloc(&s->tile_flag, s->total_tiles));
+    HIP_TRY(s, dalloc(&s->tile_scene, s->total_tiles));
+    HIP_TRY(s, dalloc(&s->list[0], s->total_tiles));
+    HIP_TRY(s, dalloc(&s->list[1], s->total_tiles));
+    HIP_TRY(s, hipMemsetAsync(s->gmp, 0, cells * sizeof(float4), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->gv, 0, cells * sizeof(float4), s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->tile_flag, 0, (size_t)s->total_tiles * 4, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> ts(s->total_tiles);
+    for (uint32_t i = 0; i < s->n_scenes; ++i) {
+        uint32_t W = s->sh[i].W;
+        std::fill(ts.begin() + s->sh[i].tile_base, ts.begin() + s->sh[i].tile_base + W * W * W, i);
+    }
+    HIP_TRY(s, hipMemcpy(s->tile_scene, ts.data(), ts.size() * 4, hipMemcpyHostToDevice));
+    uint32_t z[2] = {0u, 0u};
+    HIP_TRY(s, hipMemcpy(s->status + mpm::ST_COUNT0, z, sizeof(z), hipMemcpyHostToDevice));
+    s->grid_dirty = false;
+    return MPM_OK;
+}
+
+int clear_grid(mpm_sim* s) {
+    if (!s->grid_dirty) return MPM_OK;
+    if (s->dry) {
+        s->grid_dirty = false;
+        return MPM_OK;
+    }
+    HIP_TRY(s, mpm::launch_clear_tiles(s->gmp, s->tile_flag, s->list[s->list_cur],
+                                       s->status + mpm::ST_COUNT0 + s->list_cur, grid_blocks(s), s->stream));
+    s->grid_dirty = false;
+    return MPM_OK;
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+// keys[id] -> stable sort -> reorder particle planes into tile order.
+int sort_particles(mpm_sim* s, bool reorder) {
+    if (s->dry) {
+        if (reorder) {
+            s->ps_cur = 1 - s->ps_cur;
+            s->sorted = true;
+            s->since_sort = 0;
+            s->sorts++;
+        }
+        return MPM_OK;
+    }
+    Stage st(s, MPM_STAGE_SORT);
+    mpm::StepArgs a = args(s, 0.f, s->list_cur);
+    HIP_TRY(s, mpm::launch_tile_keys(a, s->keys, s->stream));
+    unsigned blocks = std::min<uint32_t>((s->n + 255) / 256, 8192u);
+    hipLaunchKernelGGL(k_iota, dim3(std::max(blocks, 1u)), dim3(256), 0, s->stream, s->vals, s->n);
+    HIP_TRY(s, hipGetLastError());
+    const int bits = int(ceil_log2(std::max(s->total_tiles, 2u)));
+    HIP_TRY(s, mpm::radix_sort_pairs(s->keys, s->vals, s->n, bits, s->sort_scratch, s->stream));
+    if (reorder) {
+        const int other = 1 - s->ps_cur;
+        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->vals, s->slot_of_id, s->n,
+                                       s->stream));
+        HIP_TRY(s, mpm::launch_index_slots(s->ps[other], s->cap, s->slot_of_id, s->n, s->stream));
+        s->ps_cur = other;
+        s->sorted = true;
+        s->since_sort = 0;
+        s->sorts++;
+    }
+    return MPM_OK;
+}
+
+int particle_pass(mpm_sim* s, int mode, float dt, int list_out) {
+    if (s->dry) return MPM_OK;
+    Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
+    mpm::StepArgs a = args(s, dt, list_out);
+    HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, particle_blocks(s), s->stream));
+    return MPM_OK;
+}
+
+int grid_update(mpm_sim* s, float dt, bool zero_after) {
+    if (s->dry) return MPM_OK;
+    Stage st(s, MPM_STAGE_GRID);
+    const int nxt = 1 - s->list_cur;
+    HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, s->tile_scene, s->gmp, s->gv, s->tile_flag,
+                                       s->list[s->list_cur], s->status + mpm::ST_COUNT0 + s->list_cur,
+                                       zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, s->status, dt,
+                                       zero_after ? 1 : 0, 0, grid_blocks(s), s->stream));
+    return MPM_OK;
+}
+
+int deterministic_p2g(mpm_sim* s, float dt) {
+    if (s->dry) return MPM_OK;
+    Stage st(s, MPM_STAGE_P2G);
+    if (!s->tile_begin) {
+        HIP_TRY(s, dalloc(&s->tile_begin, s->total_tiles));
+        HIP_TRY(s, dalloc(&s->tile_end, s->total_tiles));
+        HIP_TRY(s, dalloc(&s->affine, (size_t)12 * s->cap));
+    }
+    HIP_TRY(s, mpm::launch_fill_u32(s->tile_begin, 0u, s->total_tiles, s->stream));
+    HIP_TRY(s, mpm::launch_fill_u32(s->tile_end, 0u, s->total_tiles, s->stream));
+    HIP_TRY(s, mpm::launch_tile_ranges(s->keys, s->n, s->tile_begin, s->tile_end, s->stream));
+    uint32_t* cnt = s->status + mpm::ST_COUNT0 + s->list_cur;
+    HIP_TRY(s, mpm::launch_mark_dilated(s->d_scenes, s->tile_scene, s->keys, s->n, s->tile_flag,
+                                        s->list[s->list_cur], cnt, s->stream));
+    mpm::StepArgs a = args(s, dt, s->list_cur);
+    HIP_TRY(s, mpm::launch_gather_prepare(a, s->affine, particle_blocks(s), s->stream));
+    HIP_TRY(s, mpm::launch_gather_p2g(a, s->tile_scene, s->tile_begin, s->tile_end, s->affine,
+                                      s->list[s->list_cur], cnt, 4096, s->stream));
+    return MPM_OK;
+}
+
+// The substep sequence of one step_n call (see DESIGN.md "Stepping").
+int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
+    int rc;
+    if (!s->dry && (rc = ensure_grid(s)) != MPM_OK) return rc;
+    if ((rc = clear_grid(s)) != MPM_OK) return rc;
+    const bool det = s->p2g_mode == MPM_P2G_DETERMINISTIC;
+    const bool fused = s->p2g_mode == MPM_P2G_FUSED;
+    if (det) {
+        for (uint32_t k = 0; k < substeps; ++k) {
+            const bool last = k + 1 == substeps;
+            if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
+            if ((rc = deterministic_p2g(s, dt)) != MPM_OK) return rc;
+            if ((rc = grid_update(s, dt, !last)) != MPM_OK) return rc;
+            if ((rc = particle_pass(s, mpm::MODE_G2P, dt, 1 - s->list_cur)) != MPM_OK) return rc;
+            if (!last) s->list_cur = 1 - s->list_cur;
+        }
+        s->grid_dirty = true;
+        return MPM_OK;
+    }
+    if (fused && (!s->sorted || (s->sort_every == 0) || s->since_sort >= s->sort_every)) {
+        if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
+    }
+    // prologue P2G of the current state
+    if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur)) != MPM_OK) return rc;
+    for (uint32_t k = 0; k < substeps; ++k) {
+        const bool last = k + 1 == substeps;
+        if ((rc = grid_update(s, dt, !last)) != MPM_OK) return rc;
+        if (fused) {
+            if (!last && s->sort_every > 0 && s->since_sort >= s->sort_every) {
+                if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
+            }
+            if ((rc = particle_pass(s, last ? mpm::MODE_G2P : mpm::MODE_G2P2G, dt, 1 - s->list_cur)) != MPM_OK)
+                return rc;
+        } else {
+            if ((rc = particle_pass(s, mpm::MODE_G2P, dt, 1 - s->list_cur)) != MPM_OK) return rc;
+            if (!last && (rc = particle_pass(s, mpm::MODE_P2G, dt, 1 - s->list_cur)) != MPM_OK) return rc;
+        }
+        s->since_sort++;
+        if (!last) s->list_cur = 1 - s->list_cur;
+    }
+    s->grid_dirty = true;
+    return MPM_OK;
+}
+
+int field_layout(int field, int* comp0, int* ncomp) {
+    switch (field) {
+        case MPM_FIELD_POSITION: *comp0 = mpm::CX; *ncomp = 3; return 0;
+        case MPM_FIELD_VELOCITY: *comp0 = mpm::CV; *ncomp = 3; return 0;
+        case MPM_FIELD_C: *comp0 = mpm::CC; *ncomp = 9; return 0;
+        case MPM_FIELD_F: *comp0 = mpm::CF; *ncomp = 9; return 0;
+        case MPM_FIELD_JP: *comp0 = mpm::CJP; *ncomp = 1; return 0;
+        case MPM_FIELD_MASS: *comp0 = mpm::CMASS; *ncomp = 1; return 0;
+        case MPM_FIELD_VOLUME: *comp0 = mpm::CVOL; *ncomp = 1; return 0;
+        case MPM_FIELD_YOUNGS: *comp0 = mpm::CE; *ncomp = 1; return 0;
+        case MPM_FIELD_MATERIAL: *comp0 = mpm::CMAT; *ncomp = 1; return 0;
+        default: return -1;
+    }
+}
+
+int build_chunks(mpm_sim* s) {
+    std::vector<Chunk> ch;
+    for (uint32_t sc = 0; sc < s->n_scenes; ++sc) {
+        for (uint32_t off = 0; off < s->counts[sc]; off += CHUNK) {
+            Chunk c{};
+            c.scene = sc;
+            c.start = s->pbase[sc] + off;
+            c.count = std::min(CHUNK, s->counts[sc] - off);
+            ch.push_back(c);
+        }
+    }
+    s->n_chunks = uint32_t(ch.size());
+    if (!ch.empty())
+        HIP_TRY(s, hipMemcpy(s->d_chunks, ch.data(), ch.size() * sizeof(Chunk), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int upload_scenes(mpm_sim* s) {
+    std::vector<float> mats((size_t)s->n_scenes * MPM_MAX_MATERIALS * mpm::MAT_STRIDE, 0.f);
+    for (uint32_t i = 0; i < s->n_scenes; ++i) {
+        s->sh[i].pbase = s->pbase[i];
+        s->sh[i].n = s->counts[i];
+        for (uint32_t r = 0; r < s->cfg[i].n_materials; ++r)
+            std::memcpy(&mats[((size_t)i * MPM_MAX_MATERIALS + r) * mpm::MAT_STRIDE], &s->cfg[i].materials[r],
+                        sizeof(mpm_material));
+    }
+    HIP_TRY(s, hipMemcpy(s->d_scenes, s->sh.data(), s->sh.size() * sizeof(SceneDev), hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemcpy(s->d_mats, mats.data(), mats.size() * sizeof(float), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int create(const mpm_scene_config* cfgs, uint32_t n_scenes, mpm_sim** out) {
+    if (!out) return MPM_ERR_INVALID;
+    *out = nullptr;
+    if (!cfgs || n_scenes == 0) return MPM_ERR_INVALID;
+    mpm_sim* s = new mpm_sim();
+    s->n_scenes = n_scenes;
+    s->cfg.assign(cfgs, cfgs + n_scenes);
+    uint64_t tiles = 0, cap = 0;
+    for (uint32_t i = 0; i < n_scenes; ++i) {
+        std::string v = validate(cfgs[i]);
+        if (!v.empty()) {
+            *out = s;
+            return fail(s, MPM_ERR_INVALID, "scene " + std::to_string(i) + ": " + v);
+        }
+        if (cfgs[i].model != cfgs[0].model) {
+            *out = s;
+            return fail(s, MPM_ERR_INVALID, "all scenes of a batch must use the same model");
+        }
+        uint64_t W = (cfgs[i].grid_size + 3u) / 4u;
+        s->sh.push_back(make_scene(cfgs[i], uint32_t(tiles), i * MPM_MAX_MATERIALS));
+        tiles += W * W * W;
+        s->pbase.push_back(uint32_t(cap));
+        cap += cfgs[i].max_particles;
+    }
+    if (tiles >= (1ull << 26) || cap >= (1ull << 31) || cap == 0) {
+        *out = s;
+        return fail(s, MPM_ERR_INVALID, "grid or particle capacity too large");
+    }
+    s->total_tiles = uint32_t(tiles);
+    s->cap = uint32_t(cap);
+    s->counts.assign(n_scenes, 0u);
+    *out = s;
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) return hip_fail(s, e, "hipGetDevice");
+    HIP_TRY(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const size_t cells = (size_t)s->total_tiles * 64;
+    HIP_TRY(s, dalloc(&s->d_scenes, n_scenes));
+    HIP_TRY(s, dalloc(&s->d_mats, (size_t)n_scenes * MPM_MAX_MATERIALS * mpm::MAT_STRIDE));
+    HIP_TRY(s, dalloc(&s->ps[0], (size_t)mpm::NCOMP * s->cap));
+    HIP_TRY(s, dalloc(&s->ps[1], (size_t)mpm::NCOMP * s->cap));
+    HIP_TRY(s, dalloc(&s->status, mpm::ST_WORDS));
+    HIP_TRY(s, dalloc(&s->slot_of_id, s->cap));
+    HIP_TRY(s, dalloc(&s->keys, s->cap));
+    HIP_TRY(s, dalloc(&s->vals, s->cap));
+    HIP_TRY(s, dalloc(&s->sort_scratch, mpm::radix_scratch_words(s->cap)));
+    HIP_TRY(s, dalloc(&s->d_chunks, (size_t)(s->cap / CHUNK + n_scenes + 1)));
+    HIP_TRY(s, dalloc(&s->staging, (size_t)9 * s->cap));
+    HIP_TRY(s, hipMemsetAsync(s->status, 0, mpm::ST_WORDS * 4, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->ps[0], 0, (size_t)mpm::NCOMP * s->cap * 4, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    (void)cells;
+    int rc = upload_scenes(s);
+    if (rc != MPM_OK) return rc;
+    return MPM_OK;
+}
+
+int set_particles(mpm_sim* s, const uint32_t* counts, const float* pos, const float* vel) {
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < s->n_scenes; ++i) {
+        if (counts[i] > s->cfg[i].max_particles)
+            return fail(s, MPM_ERR_INVALID, "set_particles: n=" + std::to_string(counts[i]) +
+                                                " > max_particles=" + std::to_string(s->cfg[i].max_particles));
+        total += counts[i];
+    }
+    if (total > 0 && !pos) return fail(s, MPM_ERR_INVALID, "positions must not be NULL");
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->n = uint32_t(total);
+    s->counts.assign(counts, counts + s->n_scenes);
+    s->pbase.clear();
+    uint32_t acc = 0;
+    for (uint32_t i = 0; i < s->n_scenes; ++i) {
+        s->pbase.push_back(acc);
+        acc += counts[i];
+    }
+    // host-built planes [0, n) of every component
+    std::vector<float> plane(std::max<size_t>(s->n, 1));
+    float* ps = s->ps[0];
+    auto put = [&](int comp) -> hipError_t {
+        return hipMemcpy(ps + (size_t)comp * s->cap, plane.data(), (size_t)s->n * 4, hipMemcpyHostToDevice);
+    };
+    for (int k = 0; k < 3; ++k) {
+        for (uint32_t p = 0; p < s->n; ++p) plane[p] = pos[3 * (size_t)p + k];
+        HIP_TRY(s, put(mpm::CX + k));
+        for (uint32_t p = 0; p < s->n; ++p) plane[p] = vel ? vel[3 * (size_t)p + k] : 0.0f;
+        HIP_TRY(s, put(mpm::CV + k));
+    }
+    std::fill(plane.begin(), plane.end(), 0.0f);
+    for (int k = 0; k < 9; ++k) HIP_TRY(s, put(mpm::CC + k));
+    HIP_TRY(s, put(mpm::CJP));
+    HIP_TRY(s, put(mpm::CMAT));
+    for (int k = 0; k < 9; ++k) {
+        std::fill(plane.begin(), plane.end(), (k % 4 == 0) ? 1.0f : 0.0f);
+        HIP_TRY(s, put(mpm::CF + k));
+    }
+    auto per_scene = [&](auto getter) {
+        for (uint32_t sc = 0; sc < s->n_scenes; ++sc)
+            for (uint32_t p = 0; p < counts[sc]; ++p) plane[s->pbase[sc] + p] = getter(s->cfg[sc]);
+    };
+    per_scene([](const mpm_scene_config& c) { return c.particle_mass; });
+    HIP_TRY(s, put(mpm::CMASS));
+    per_scene([](const mpm_scene_config& c) { return c.particle_volume; });
+    HIP_TRY(s, put(mpm::CVOL));
+    per_scene([](const mpm_scene_config& c) { return c.youngs; });
+    HIP_TRY(s, put(mpm::CE));
+    for (uint32_t p = 0; p < s->n; ++p) {
+        uint32_t id = p;
+        std::memcpy(&plane[p], &id, 4);
+    }
+    HIP_TRY(s, put(mpm::CID));
+    s->ps_cur = 0;
+    HIP_TRY(s, mpm::launch_index_slots(s->ps[0], s->cap, s->slot_of_id, s->n, s->stream));
+    int rc = build_chunks(s);
+    if (rc != MPM_OK) return rc;
+    rc = upload_scenes(s);
+    if (rc != MPM_OK) return rc;
+    s->has_E = 0;
+    s->sorted = false;
+    s->diff_steps = 0;
+    if (s->gmp) {
+        rc = clear_grid(s);
+        if (rc != MPM_OK) return rc;
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_abi_version(void) { return MPM_ABI_VERSION; }
+
+int mpm_create(const mpm_scene_config* cfg, mpm_sim** out) { return create(cfg, 1, out); }
+
+int mpm_create_batch(const mpm_scene_config* cfgs, uint32_t n_scenes, mpm_sim** out) {
+    return create(cfgs, n_scenes, out);
+}
+
+int mpm_destroy(mpm_sim* sim) {
+    if (!sim) return MPM_ERR_INVALID;
+    release(sim);
+    delete sim;
+    return MPM_OK;
+}
+
+const char* mpm_last_error(const mpm_sim* sim) {
+    if (!sim) return "null handle";
+    return sim->err.c_str();
+}
+
+int mpm_set_particles(mpm_sim* sim, const float* positions, const float* velocities, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (sim->n_scenes != 1) return fail(sim, MPM_ERR_INVALID, "use mpm_set_particles_batch for batches");
+    return set_particles(sim, &n, positions, velocities);
+}
+
+int mpm_set_particles_batch(mpm_sim* sim, const uint32_t* counts, const float* positions,
+                            const float* velocities) {
+    if (!sim || !counts) return MPM_ERR_INVALID;
+    return set_particles(sim, counts, positions, velocities);
+}
+
+int mpm_write_field(mpm_sim* sim, int field, const void* data, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int c0, nc;
+    if (field_layout(field, &c0, &nc) != 0) return fail(sim, MPM_ERR_INVALID, "unknown field");
+    if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "field length must equal the particle count");
+    if (n == 0) return MPM_OK;
+    if (!data) return fail(sim, MPM_ERR_INVALID, "data must not be NULL");
+    HIP_TRY(sim, hipMemcpyAsync(sim->staging, data, (size_t)n * nc * 4, hipMemcpyHostToDevice, sim->stream));
+    HIP_TRY(sim, mpm::launch_permute_in(sim->ps[sim->ps_cur], sim->cap, c0, nc, sim->slot_of_id, sim->staging, n,
+                                        sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    if (field == MPM_FIELD_YOUNGS) sim->has_E = 1;
+    if (field == MPM_FIELD_POSITION) sim->sorted = false;
+    sim->diff_steps = 0;
+    return MPM_OK;
+}
+
+int mpm_read_field(mpm_sim* sim, int field, void* out, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int c0, nc;
+    if (field_layout(field, &c0, &nc) != 0) return fail(sim, MPM_ERR_INVALID, "unknown field");
+    if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "output length must equal the particle count");
+    if (n == 0) return MPM_OK;
+    if (!out) return fail(sim, MPM_ERR_INVALID, "out must not be NULL");
+    HIP_TRY(sim, mpm::launch_unpermute(sim->ps[sim->ps_cur], sim->cap, c0, nc, sim->slot_of_id, sim->staging, n,
+                                       sim->stream));
+    HIP_TRY(sim, hipMemcpyAsync(out, sim->staging, (size_t)n * nc * 4, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    flush_profile(sim);
+    return MPM_OK;
+}
+
+int mpm_read_positions(mpm_sim* sim, float* out, uint32_t n) {
+    return mpm_read_field(sim, MPM_FIELD_POSITION, out, n);
+}
+int mpm_read_velocities(mpm_sim* sim, float* out, uint32_t n) {
+    return mpm_read_field(sim, MPM_FIELD_VELOCITY, out, n);
+}
+
+int mpm_set_materials(mpm_sim* sim, uint32_t scene, const mpm_material* rows, uint32_t n_rows) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    if (!rows || n_rows < 1 || n_rows > MPM_MAX_MATERIALS)
+        return fail(sim, MPM_ERR_INVALID, "materials: 1..8 rows required");
+    for (uint32_t r = 0; r < n_rows; ++r) sim->cfg[scene].materials[r] = rows[r];
+    sim->cfg[scene].n_materials = n_rows;
+    sim->sh[scene].n_mat = n_rows;
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    sim->graphs.clear();
+    return upload_scenes(sim);
+}
+
+int mpm_set_scene_params(mpm_sim* sim, uint32_t scene, const mpm_scene_config* cfg) {
+    if (!sim || !cfg) return MPM_ERR_INVALID;
+    if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    std::string v = validate(*cfg);
+    if (!v.empty()) return fail(sim, MPM_ERR_INVALID, v);
+    const mpm_scene_config& old = sim->cfg[scene];
+    if (cfg->grid_size != old.grid_size || cfg->max_particles != old.max_particles || cfg->model != old.model)
+        return fail(sim, MPM_ERR_INVALID, "grid_size, max_particles and model are fixed at creation");
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    SceneDev d = make_scene(*cfg, sim->sh[scene].tile_base, sim->sh[scene].mat_base);
+    sim->cfg[scene] = *cfg;
+    sim->sh[scene] = d;
+    for (auto& kv : sim->graphs) hipGraphExecDestroy(kv.second);
+    sim->graphs.clear();
+    sim->sorted = false;
+    return upload_scenes(sim);
+}
+
+int mpm_set_p2g_mode(mpm_sim* sim, int mode) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (mode < 0 || mode > 2) return fail(sim, MPM_ERR_INVALID, "unknown p2g mode");
+    if (mode == MPM_P2G_DETERMINISTIC && sim->sh[0].model != MPM_MODEL_SHADER)
+        return fail(sim, MPM_ERR_INVALID, "deterministic P2G is defined for the shader model");
+    sim->p2g_mode = mode;
+    return MPM_OK;
+}
+
+int mpm_set_sort_interval(mpm_sim* sim, uint32_t every) {
+    if (!sim) return MPM_ERR_INVALID;
+    sim->sort_every = every;
+    return MPM_OK;
+}
+
+int mpm_set_write_vc(mpm_sim* sim, int every_substep) {
+    if (!sim) return MPM_ERR_INVALID;
+    sim->write_vc = every_substep ? 1 : 0;
+    return MPM_OK;
+}
+
+int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks_) {
+    if (!sim) return MPM_ERR_INVALID;
+    sim->pblocks = particle_blocks;
+    sim->gblocks = grid_blocks_;
+    return MPM_OK;
+}
+
+int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!std::isfinite(dt) || dt < 0.0f) return fail(sim, MPM_ERR_INVALID, "dt must be finite and >= 0");
+    if (sim->n == 0 || substeps == 0) return MPM_OK;
+    sim->diff_steps = 0;
+    return run_step_n(sim, dt, substeps);
+}
+
+int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!std::isfinite(dt) || dt < 0.0f) return fail(sim, MPM_ERR_INVALID, "dt must be finite and >= 0");
+    if (sim->n == 0 || substeps == 0) return MPM_OK;
+    sim->diff_steps = 0;
+    uint32_t dtb;
+    std::memcpy(&dtb, &dt, 4);
+    const bool sort_start = sim->p2g_mode == MPM_P2G_FUSED &&
+                            (!sim->sorted || sim->sort_every == 0 || sim->since_sort >= sim->sort_every);
+    GraphKey key{dtb, substeps, uint32_t(sim->p2g_mode), uint32_t(sim->ps_cur), uint32_t(sim->list_cur),
+                 uint32_t(sim->grid_dirty), uint32_t(sort_start) | (sim->since_sort << 1), uint32_t(sim->write_vc),
+                 sim->pblocks, sim->gblocks};
+    auto it = sim->graphs.find(key);
+    if (it == sim->graphs.end()) {
+        // Lazily allocated tables must exist before capture.
+        {
+            int rc0 = ensure_grid(sim);
+            if (rc0 != MPM_OK) return rc0;
+        }
+        if (sim->p2g_mode == MPM_P2G_DETERMINISTIC && !sim->tile_begin) {
+            HIP_TRY(sim, dalloc(&sim->tile_begin, sim->total_tiles));
+            HIP_TRY(sim, dalloc(&sim->tile_end, sim->total_tiles));
+            HIP_TRY(sim, dalloc(&sim->affine, (size_t)12 * sim->cap));
+        }
+        hipGraph_t g;
+        HIP_TRY(sim, hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
+        sim->capturing = true;
+        int rc = run_step_n(sim, dt, substeps);
+        sim->capturing = false;
+        hipError_t e = hipStreamEndCapture(sim->stream, &g);
+        if (rc != MPM_OK) return rc;
+        if (e != hipSuccess) return hip_fail(sim, e, "hipStreamEndCapture");
+        hipGraphExec_t ex;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) return hip_fail(sim, e, "hipGraphInstantiate");
+        it = sim->graphs.emplace(key, ex).first;
+    } else {
+        sim->dry = true;
+        int rc = run_step_n(sim, dt, substeps);
+        sim->dry = false;
+        if (rc != MPM_OK) return rc;
+    }
+    HIP_TRY(sim, hipGraphLaunch(it->second, sim->stream));
+    return MPM_OK;
+}
+
+int mpm_synchronize(mpm_sim* sim) {
+    if (!sim) return MPM_ERR_INVALID;
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    flush_profile(sim);
+    return MPM_OK;
+}
+
+int mpm_read_grid_mass_momentum(mpm_sim* sim, uint32_t scene, float* mass, float* momentum) {
+    if (!sim || !mass || !momentum) return MPM_ERR_INVALID;
+    if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    const uint32_t G = sim->cfg[scene].grid_size;
+    const size_t cells = (size_t)G * G * G;
+    {
+        int rc0 = ensure_grid(sim);
+        if (rc0 != MPM_OK) return rc0;
+    }
+    float* buf = nullptr;
+    HIP_TRY(sim, hipMalloc((void**)&buf, cells * 4 * sizeof(float)));
+    hipError_t e = mpm::launch_grid_to_dense(sim->gmp, sim->d_scenes, scene, G, buf, buf + cells, sim->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(mass, buf, cells * 4, hipMemcpyDeviceToHost, sim->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(momentum, buf + cells, cells * 12, hipMemcpyDeviceToHost, sim->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+    hipFree(buf);
+    if (e != hipSuccess) return hip_fail(sim, e, "read_grid_mass_momentum");
+    return MPM_OK;
+}
+
+int mpm_read_status(mpm_sim* sim, mpm_status_info* out) {
+    if (!sim || !out) return MPM_ERR_INVALID;
+    uint32_t w[mpm::ST_WORDS];
+    HIP_TRY(sim, hipMemcpyAsync(w, sim->status, sizeof(w), hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    out->fault_bits = w[mpm::ST_FAULT];
+    out->completed_gathers = w[mpm::ST_GATHERS];
+    out->active_tiles = w[mpm::ST_COUNT0 + sim->list_cur];
+    out->sorts = sim->sorts;
+    return MPM_OK;
+}
+
+int mpm_reset_status(mpm_sim* sim) {
+    if (!sim) return MPM_ERR_INVALID;
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    uint32_t z[2] = {0u, 0u};
+    HIP_TRY(sim, hipMemcpy(sim->status, z, sizeof(z), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_t* keys, uint32_t* ids) {
+    if (!sim || !keys || !ids) return MPM_ERR_INVALID;
+    if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    const uint32_t cnt = sim->counts[scene];
+    if (capacity < cnt) return fail(sim, MPM_ERR_INVALID, "capacity below the particle count");
+    if (sim->n > 0) {
+        int rc = sort_particles(sim, false);
+        if (rc != MPM_OK) return rc;
+        HIP_TRY(sim, hipMemcpyAsync(keys, sim->keys + sim->pbase[scene], (size_t)cnt * 4, hipMemcpyDeviceToHost,
+                                    sim->stream));
+        HIP_TRY(sim, hipMemcpyAsync(ids, sim->vals + sim->pbase[scene], (size_t)cnt * 4, hipMemcpyDeviceToHost,
+                                    sim->stream));
+        HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    }
+    const uint32_t tb = sim->sh[scene].tile_base, pb = sim->pbase[scene];
+    for (uint32_t i = 0; i < cnt; ++i) {
+        keys[i] -= tb;
+        ids[i] -= pb;
+    }
+    for (uint32_t i = cnt; i < capacity; ++i) {
+        keys[i] = 0xffffffffu;
+        ids[i] = i;
+    }
+    return MPM_OK;
+}
+
+int mpm_set_profiling(mpm_sim* sim, int enabled) {
+    if (!sim) return MPM_ERR_INVALID;
+    sim->profiling = enabled != 0;
+    return MPM_OK;
+}
+
+int mpm_read_profile(mpm_sim* sim, float* ms, uint32_t* launches) {
+    if (!sim) return MPM_ERR_INVALID;
+    flush_profile(sim);
+    for (int i = 0; i < MPM_STAGE_COUNT; ++i) {
+        if (ms) ms[i] = float(sim->prof_ms[i]);
+        if (launches) launches[i] = sim->prof_n[i];
+    }
+    return MPM_OK;
+}
+
+int mpm_reset_profile(mpm_sim* sim) {
+    if (!sim) return MPM_ERR_INVALID;
+    flush_profile(sim);
+    for (int i = 0; i < MPM_STAGE_COUNT; ++i) {
+        sim->prof_ms[i] = 0.0;
+        sim->prof_n[i] = 0;
+    }
+    return MPM_OK;
+}
+
+int mpm_diff_forward(mpm_sim* sim, float dt, uint32_t steps, uint32_t checkpoint_every) {
+    if (!sim) return MPM_ERR_INVALID;
+    (void)dt;
+    (void)steps;
+    (void)checkpoint_every;
+    return fail(sim, MPM_ERR_INVALID, "differentiable path not built in this library version");
+}
+
+int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad_v_final, float* grad_x0,
+                      float* grad_v0, float* grad_E, double* grad_mu_lambda) {
+    if (!sim) return MPM_ERR_INVALID;
+    (void)grad_x_final;
+    (void)grad_v_final;
+    (void)grad_x0;
+    (void)grad_v0;
+    (void)grad_E;
+    (void)grad_mu_lambda;
+    return fail(sim, MPM_ERR_INVALID, "differentiable path not built in this library version");
+}
+
+}  // extern "C"

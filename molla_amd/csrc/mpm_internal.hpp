@@ -1,0 +1,134 @@
+// mpm_internal.hpp -- structures shared by the host side and the kernels of
+// the MI355X MPM substep library.
+//
+// Storage layout in HBM (all device-resident for the life of a context):
+//   * Particles: component-major SoA, NCOMP float planes of length `cap`
+//     (plane k at ps + k * cap).  Planes: x(0-2) v(3-5) C(6-14, row-major)
+//     F(15-23, row-major) jp(24) mass(25) vol(26) E(27) mat(28, u32 bits)
+//     id(29, u32 bits: original particle index).  Two copies (ping-pong) so
+//     the tile reorder is a pure gather.
+//   * Grid: dense, tile-major.  Every scene owns W^3 tiles of 4x4x4 cells
+//     (W = ceil(G/4)) starting at its `tile_base`; cell (ix,iy,iz) of scene s
+//     lives at ((tile_base + tx + W*(ty + W*tz)) << 6) | lx | ly<<2 | lz<<4.
+//     `gmp` holds the P2G accumulation (m, px, py, pz) and `gv` the grid
+//     velocity (vx, vy, vz, m) after the grid update.  Only tiles on the
+//     active list are touched in a substep.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpm {
+
+constexpr int NCOMP = 30;
+enum Comp : int {
+    CX = 0, CV = 3, CC = 6, CF = 15, CJP = 24, CMASS = 25, CVOL = 26, CE = 27, CMAT = 28, CID = 29
+};
+
+constexpr int MODEL_SHADER = 0;
+constexpr int MODEL_CPUDIFF = 1;
+
+constexpr int RHEO_SAND = 0, RHEO_SNOW = 1, RHEO_VISCOUS = 2, RHEO_FLUID = 3;
+
+constexpr uint32_t FAULT_OVERFLOW = 1u;
+constexpr uint32_t FAULT_NUMERICAL = 2u;
+
+// Status words in device memory.
+enum StatusWord : int { ST_FAULT = 0, ST_GATHERS = 1, ST_COUNT0 = 2, ST_COUNT1 = 3, ST_WORDS = 8 };
+
+struct SceneDev {
+    float origin[3];
+    float dx;
+    float inv_dx;
+    float gravity_y;
+    float youngs;
+    float poisson;
+    float boundary_friction;
+    float floor_y;
+    float floor_friction;
+    float mu;   // youngs / (2 (1 + nu)), f32 like mpm.wgsl:542
+    float lam;  // youngs nu / ((1 + nu)(1 - 2 nu)), mpm.wgsl:543
+    uint32_t G;
+    uint32_t W;
+    uint32_t tile_base;
+    int32_t window[3];
+    uint32_t model;
+    uint32_t mat_base;
+    uint32_t n_mat;
+    uint32_t pbase;
+    uint32_t n;
+};
+
+// A contiguous run of particle slots of one scene, processed by one block.
+struct Chunk {
+    uint32_t scene;
+    uint32_t start;
+    uint32_t count;
+    uint32_t pad;
+};
+
+constexpr int MAT_STRIDE = 8;  // rheology, dp_alpha, theta_c, theta_s, hardening, vdamp, adamp, bulk
+
+enum Mode : int { MODE_P2G = 0, MODE_G2P = 1, MODE_G2P2G = 2 };
+
+struct StepArgs {
+    const SceneDev* scenes;
+    const float* mats;
+    float* ps;  // particle planes
+    uint32_t cap;
+    float4* gmp;
+    float4* gv;
+    uint32_t* tile_flag;
+    uint32_t* list_out;   // active-tile list being built by the scatter
+    uint32_t* count_out;  // its counter
+    uint32_t* status;
+    const Chunk* chunks;
+    uint32_t n_chunks;
+    float dt;
+    int use_lds;   // 0: every node goes straight to global atomics
+    int write_vc;  // 1: store v and C every substep
+    int has_E;     // per-particle Young's modulus plane is valid
+};
+
+// ---- launchers (mpm_kernels.hip) ----
+hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
+                                hipStream_t s);
+hipError_t launch_grid_update(const SceneDev* scenes, const uint32_t* tile_scene, float4* gmp,
+                              float4* gv, uint32_t* tile_flag, const uint32_t* list,
+                              const uint32_t* count, uint32_t* next_count, uint32_t* status,
+                              float dt, int zero_after, int count_gather, int blocks,
+                              hipStream_t s);
+hipError_t launch_clear_tiles(float4* gmp, uint32_t* tile_flag, const uint32_t* list,
+                              uint32_t* count, int blocks, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t s);
+hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* sorted_ids,
+                          const uint32_t* slot_of_id, uint32_t n, hipStream_t s);
+hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,
+                              hipStream_t s);
+hipError_t launch_unpermute(const float* ps, uint32_t cap, int comp0, int ncomp,
+                            const uint32_t* slot_of_id, float* out, uint32_t n, hipStream_t s);
+hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
+                             const uint32_t* slot_of_id, const float* in, uint32_t n,
+                             hipStream_t s);
+hipError_t launch_grid_to_dense(const float4* gmp, const SceneDev* scenes, uint32_t scene,
+                                uint32_t G, float* mass, float* mom, hipStream_t s);
+// Deterministic gather-order P2G.
+hipError_t launch_tile_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_begin,
+                              uint32_t* tile_end, hipStream_t s);
+hipError_t launch_mark_dilated(const SceneDev* scenes, const uint32_t* tile_scene,
+                               const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_flag,
+                               uint32_t* list, uint32_t* count, hipStream_t s);
+hipError_t launch_gather_prepare(const StepArgs& a, float* affine, int blocks, hipStream_t s);
+hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene,
+                             const uint32_t* tile_begin, const uint32_t* tile_end,
+                             const float* affine, const uint32_t* list, const uint32_t* count,
+                             int blocks, hipStream_t s);
+
+// ---- radix sort (mpm_sort.hip) ----
+size_t radix_scratch_words(uint32_t n);
+// Stable LSD sort of (key, value) pairs on the low `bits` bits of the keys.
+// keys/vals are sorted in place; scratch holds radix_scratch_words(n) u32.
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits,
+                            uint32_t* scratch, hipStream_t s);
+
+}  // namespace mpm

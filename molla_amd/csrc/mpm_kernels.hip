@@ -1,0 +1,935 @@
+// mpm_kernels.hip -- HIP kernels of the MPM substep for gfx950 (MI355X).
+//
+// The per-substep hot path is two launches:
+//   k_grid_update  over the active-tile list (mpm.wgsl grid_update :1010-1058)
+//   k_particles    G2P -> x/v/C/F update -> plasticity -> stress -> P2G, fused
+//                  (mpm.wgsl g2p :1241-1353, plasticity :1355-1479,
+//                  p2g/p2g_two_pass :494-925) so a particle's state is read
+//                  once and written once per substep and the polar/SVD of F is
+//                  computed once per substep (FP64).
+// The P2G half stages the block's scatter in LDS over the bounding box of its
+// particles' stencils (particles are kept sorted by 4^3 tile, so that box is
+// small), then flushes each touched cell with one global f32 atomic per
+// channel and appends newly touched tiles to the next active list.
+#include "mpm_device.hpp"
+#include "mpm_internal.hpp"
+
+namespace mpm {
+
+constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+constexpr int LDS_CELLS = 2048;  // 4 planes x 2048 x 4 B = 32 KiB per block
+
+__device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int iy, int iz) {
+    uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
+    return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
+}
+
+__device__ __forceinline__ bool in_grid(const SceneDev& sc, int ix, int iy, int iz) {
+    int n = int(sc.G);
+    return ix >= 0 && iy >= 0 && iz >= 0 && ix < n && iy < n && iz < n;
+}
+
+__device__ __forceinline__ void lds_add(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void global_add(float* p, float v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void mark_tile(uint32_t* flag, uint32_t* list, uint32_t* count, uint32_t t) {
+    if (__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        if (atomicExch(&flag[t], 1u) == 0u) {
+            uint32_t slot = atomicAdd(count, 1u);
+            list[slot] = t;
+        }
+    }
+}
+
+__device__ __forceinline__ void latch(uint32_t* status, uint32_t bits) {
+    atomicOr(&status[ST_FAULT], bits);
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Material row of a particle (mpm.wgsl:152-157): global table at scene base.
+__device__ __forceinline__ const float* mat_row(const StepArgs& a, const SceneDev& sc, uint32_t m) {
+    if (m >= sc.n_mat) m = 0;
+    return a.mats + (size_t)(sc.mat_base + m) * MAT_STRIDE;
+}
+
+struct Scatter {
+    bool on;
+    int b[3];
+    float fx[3];
+    float m;
+    float mv[3];
+    float A[9];  // m C + 4/dx^2 stress
+};
+
+// P2G of one block's particles.  Every thread of the block must call this.
+__device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
+                                              float* s_acc, int* s_red, int* s_box) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float dx = sc.dx;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = p.on ? p.b[k] : 0x3fffffff;
+        hi[k] = p.on ? p.b[k] + 2 : -0x3fffffff;
+        lo[k] = wave_min(lo[k]);
+        hi[k] = wave_max(hi[k]);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s_red[wave * 6 + k] = lo[k];
+            s_red[wave * 6 + 3 + k] = hi[k];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int l[3], h[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            l[k] = s_red[k];
+            h[k] = s_red[3 + k];
+            for (int w = 1; w < WAVES; ++w) {
+                l[k] = min(l[k], s_red[w * 6 + k]);
+                h[k] = max(h[k], s_red[w * 6 + 3 + k]);
+            }
+            l[k] = max(l[k], 0);
+            h[k] = min(h[k], int(sc.G) - 1);
+        }
+        int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
+        int use = 0;
+        if (a.use_lds && bx > 0 && by > 0 && bz > 0 && bx * by * bz <= LDS_CELLS) use = 1;
+        if (bx <= 0 || by <= 0 || bz <= 0) use = 2;  // nothing to scatter
+        s_box[0] = l[0];
+        s_box[1] = l[1];
+        s_box[2] = l[2];
+        s_box[3] = bx;
+        s_box[4] = by;
+        s_box[5] = bz;
+        s_box[6] = use;
+    }
+    __syncthreads();
+    const int use = s_box[6];
+    if (use == 2) return;
+    float w[3][3];
+    if (p.on) {
+        quad_weights(p.fx[0], w[0]);
+        quad_weights(p.fx[1], w[1]);
+        quad_weights(p.fx[2], w[2]);
+    }
+    if (use == 1) {
+        const int x0 = s_box[0], y0 = s_box[1], z0 = s_box[2], bx = s_box[3], by = s_box[4];
+        const int vol = bx * by * s_box[5];
+        for (int i = tid; i < vol; i += BLOCK) {
+            s_acc[i] = 0.0f;
+            s_acc[LDS_CELLS + i] = 0.0f;
+            s_acc[2 * LDS_CELLS + i] = 0.0f;
+            s_acc[3 * LDS_CELLS + i] = 0.0f;
+        }
+        __syncthreads();
+        if (p.on) {
+            for (int k = 0; k < 3; ++k)
+                for (int j = 0; j < 3; ++j)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
+                        if (!in_grid(sc, ix, iy, iz)) continue;
+                        float wt = w[0][i] * w[1][j] * w[2][k];
+                        float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
+                              d2 = (float(k) - p.fx[2]) * dx;
+                        int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
+                        lds_add(&s_acc[li], wt * p.m);
+                        lds_add(&s_acc[LDS_CELLS + li], wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
+                        lds_add(&s_acc[2 * LDS_CELLS + li], wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
+                        lds_add(&s_acc[3 * LDS_CELLS + li], wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
+                    }
+        }
+        __syncthreads();
+        for (int i = tid; i < vol; i += BLOCK) {
+            float m = s_acc[i], px = s_acc[LDS_CELLS + i], py = s_acc[2 * LDS_CELLS + i],
+                  pz = s_acc[3 * LDS_CELLS + i];
+            if (m == 0.0f && px == 0.0f && py == 0.0f && pz == 0.0f) continue;
+            int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
+            int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
+            uint32_t c = cell_index(sc, ix, iy, iz);
+            float* g = reinterpret_cast<float*>(&a.gmp[c]);
+            global_add(g + 0, m);
+            global_add(g + 1, px);
+            global_add(g + 2, py);
+            global_add(g + 3, pz);
+            mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
+        }
+        __syncthreads();
+    } else if (p.on) {
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j)
+                for (int i = 0; i < 3; ++i) {
+                    int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
+                    if (!in_grid(sc, ix, iy, iz)) continue;
+                    float wt = w[0][i] * w[1][j] * w[2][k];
+                    float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
+                          d2 = (float(k) - p.fx[2]) * dx;
+                    uint32_t c = cell_index(sc, ix, iy, iz);
+                    float* g = reinterpret_cast<float*>(&a.gmp[c]);
+                    global_add(g + 0, wt * p.m);
+                    global_add(g + 1, wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
+                    global_add(g + 2, wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
+                    global_add(g + 3, wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
+                    mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
+                }
+    }
+}
+
+// Kirchhoff-form stress of a particle from its SVD; returns the affine
+// matrix A = m C + 4/dx^2 * (-dt vol U diag(tau) U^T).
+template <int MODEL>
+__device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float* mrow, const Svd32& d,
+                                                float mu, float lam, float jp, float dt, float vol,
+                                                float m, const float* C, float* A) {
+    float tau[3];
+    if (MODEL == MODEL_SHADER) {
+        int rheo = int(mrow[0] + 0.5f);
+        float h = 1.0f;
+        if (rheo == RHEO_SNOW) h = __expf(clampf(-mrow[4] * jp, -3.0f, 3.0f));
+        else if (rheo == RHEO_SAND) h = __expf(clampf(jp, -3.0f, 3.0f));
+        float mu_h = mu * h, lam_h = lam * h;
+        if (rheo == RHEO_FLUID) {
+            mu_h = 0.0f;
+            lam_h = mrow[7];
+        }
+        float s0 = clampf(d.s[0], 0.1f, 10.0f), s1 = clampf(d.s[1], 0.1f, 10.0f),
+              s2 = clampf(d.s[2], 0.1f, 10.0f);
+        float J = s0 * s1 * s2;
+        float vj = lam_h * (J - 1.0f) * J;
+        tau[0] = 2.0f * mu_h * (s0 - 1.0f) * s0 + vj;
+        tau[1] = 2.0f * mu_h * (s1 - 1.0f) * s1 + vj;
+        tau[2] = 2.0f * mu_h * (s2 - 1.0f) * s2 + vj;
+    } else {
+        float tr = d.s[0] + d.s[1] + d.s[2] - 3.0f;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tau[i] = (2.0f * mu * (d.s[i] - 1.0f) + lam * tr) * d.s[i];
+    }
+    float S[9];
+    udut(d.U, tau, S);
+    const float k = -dt * vol * 4.0f * sc.inv_dx * sc.inv_dx;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) A[i] = m * C[i] + k * S[i];
+}
+
+template <int MODE, int MODEL>
+__global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
+    __shared__ float s_acc[4 * LDS_CELLS];
+    __shared__ int s_red[WAVES * 6];
+    __shared__ int s_box[8];
+    if (threadIdx.x == 0) s_box[7] = a.status[ST_FAULT] != 0u ? 1 : 0;
+    __syncthreads();
+    if (s_box[7]) return;  // block-uniform: a latched fault stops the pass
+    if (MODE != MODE_P2G && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.status[ST_GATHERS], 1u);
+    const uint32_t cap = a.cap;
+    float* __restrict__ ps = a.ps;
+    const float dt = a.dt;
+    for (uint32_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+        const Chunk ch = a.chunks[ci];
+        const SceneDev sc = a.scenes[ch.scene];
+        const uint32_t slot = ch.start + threadIdx.x;
+        const bool active = threadIdx.x < ch.count;
+        Scatter sp;
+        sp.on = false;
+        if (active) {
+            float x[3], F[9];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) x[k] = ps[(size_t)(CX + k) * cap + slot];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) F[k] = ps[(size_t)(CF + k) * cap + slot];
+            float jp = ps[(size_t)CJP * cap + slot];
+            const float m = ps[(size_t)CMASS * cap + slot];
+            const float vol = ps[(size_t)CVOL * cap + slot];
+            const uint32_t matid = __float_as_uint(ps[(size_t)CMAT * cap + slot]);
+            const float* mrow = mat_row(a, sc, matid);
+            float mu = sc.mu, lam = sc.lam;
+            if (a.has_E) {
+                float E = ps[(size_t)CE * cap + slot];
+                mu = E / (2.0f * (1.0f + sc.poisson));
+                lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
+            }
+            bool ok = true;
+            float v[3], C[9];
+            if (MODE == MODE_P2G) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[k] = ps[(size_t)(CV + k) * cap + slot];
+#pragma unroll
+                for (int k = 0; k < 9; ++k) C[k] = ps[(size_t)(CC + k) * cap + slot];
+                if (MODEL == MODEL_SHADER) {
+                    bool fin = finitef(jp);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) fin = fin && finitef(x[k]) && finitef(v[k]);
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) fin = fin && finitef(C[k]) && finitef(F[k]);
+                    if (!fin) {
+                        latch(a.status, FAULT_NUMERICAL);
+                        ok = false;
+                    }
+                }
+                if (ok) {
+                    Svd32 d;
+                    svd3<MODEL>(F, d);
+                    affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp, dt, vol, m, C, sp.A);
+                }
+            } else {
+                // ---- G2P (mpm.wgsl:1241-1353 / mpm_cpu_diff.rs:544-580) ----
+                float xp[3], fx[3], w[3][3];
+                int b[3];
+                bool fin = finitef(jp);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
+                    fin = fin && finitef(x[k]) && finitef(xp[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < 9; ++k) fin = fin && finitef(F[k]);
+                if (MODEL == MODEL_SHADER && !fin) {
+                    latch(a.status, FAULT_NUMERICAL);
+                    ok = false;
+                }
+                if (ok) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        float bf = floorf(xp[k] - 0.5f);
+                        b[k] = int(bf) - (MODEL == MODEL_SHADER ? sc.window[k] : 0);
+                        fx[k] = xp[k] - bf;
+                        quad_weights(fx[k], w[k]);
+                    }
+                    float vn[3] = {0.f, 0.f, 0.f};
+                    float Cn[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) Cn[k] = 0.f;
+                    const float dx = sc.dx, s4 = 4.0f * sc.inv_dx * sc.inv_dx;
+                    for (int k = 0; k < 3; ++k)
+                        for (int j = 0; j < 3; ++j)
+#pragma unroll
+                            for (int i = 0; i < 3; ++i) {
+                                int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
+                                if (!in_grid(sc, ix, iy, iz)) continue;
+                                float4 g = a.gv[cell_index(sc, ix, iy, iz)];
+                                float wt = w[0][i] * w[1][j] * w[2][k];
+                                float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
+                                      d2 = (float(k) - fx[2]) * dx;
+                                vn[0] += wt * g.x;
+                                vn[1] += wt * g.y;
+                                vn[2] += wt * g.z;
+                                float sw = s4 * wt;
+                                float gx = sw * g.x, gy = sw * g.y, gz = sw * g.z;
+                                Cn[0] += gx * d0; Cn[1] += gx * d1; Cn[2] += gx * d2;
+                                Cn[3] += gy * d0; Cn[4] += gy * d1; Cn[5] += gy * d2;
+                                Cn[6] += gz * d0; Cn[7] += gz * d1; Cn[8] += gz * d2;
+                            }
+                    const int rheo = int(mrow[0] + 0.5f);
+                    if (MODEL == MODEL_SHADER && rheo == RHEO_VISCOUS) {
+                        float vd = __expf(-fmaxf(mrow[5], 0.0f) * dt);
+                        float ad = __expf(-fmaxf(mrow[6], 0.0f) * dt);
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) vn[k] *= vd;
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) Cn[k] *= ad;
+                    }
+                    float nx[3];
+                    bool fin2 = true;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        nx[k] = x[k] + dt * vn[k];
+                        fin2 = fin2 && finitef(nx[k]) && finitef(vn[k]);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) fin2 = fin2 && finitef(Cn[k]);
+                    if (!fin2) {
+                        latch(a.status, FAULT_NUMERICAL);
+                        ok = false;
+                    }
+                    if (ok && MODEL == MODEL_SHADER) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) {
+                            float lo = sc.origin[k] + float(sc.window[k] + 2) * dx;
+                            float hi = sc.origin[k] + float(sc.window[k] + int(sc.G) - 3) * dx;
+                            if (nx[k] < lo || nx[k] > hi) vn[k] = 0.0f;
+                            nx[k] = fminf(fmaxf(nx[k], lo), hi);
+                        }
+                    }
+                    float Fn[9];
+                    if (ok) {
+#pragma unroll
+                        for (int i = 0; i < 3; ++i)
+#pragma unroll
+                            for (int j = 0; j < 3; ++j)
+                                Fn[3 * i + j] = F[3 * i + j] + dt * (Cn[3 * i] * F[j] + Cn[3 * i + 1] * F[3 + j] +
+                                                                     Cn[3 * i + 2] * F[6 + j]);
+#pragma unroll
+                        for (int k = 0; k < 9; ++k)
+                            if (!finitef(Fn[k])) ok = false;
+                        if (!ok) latch(a.status, FAULT_NUMERICAL);
+                    }
+                    if (ok) {
+                        // ---- plasticity (mpm.wgsl:1355-1479) + stress from the same SVD ----
+                        Svd32 d;
+                        bool have_svd = false;
+                        bool plast_ok = true;
+                        float jp_new = jp;
+                        if (MODEL == MODEL_SHADER) {
+                            if (rheo == RHEO_FLUID) {
+                                float J = Fn[0] * (Fn[4] * Fn[8] - Fn[5] * Fn[7]) -
+                                          Fn[1] * (Fn[3] * Fn[8] - Fn[5] * Fn[6]) +
+                                          Fn[2] * (Fn[3] * Fn[7] - Fn[4] * Fn[6]);
+                                float s = clampf(cbrtf(J), 0.05f, 20.0f);
+                                if (!finitef(s)) {
+                                    plast_ok = false;
+                                } else {
+#pragma unroll
+                                    for (int k = 0; k < 9; ++k) Fn[k] = (k % 4 == 0) ? s : 0.0f;
+                                    jp_new = 0.0f;
+                                    d.s[0] = d.s[1] = d.s[2] = s;
+#pragma unroll
+                                    for (int k = 0; k < 9; ++k) d.U[k] = d.V[k] = (k % 4 == 0) ? 1.0f : 0.0f;
+                                    have_svd = true;
+                                }
+                            } else {
+                                svd3<MODEL_SHADER>(Fn, d);
+                                float ns[3] = {d.s[0], d.s[1], d.s[2]};
+                                float dq = 0.0f;
+                                bool jp_replace = false;
+                                bool fin3 = finitef(ns[0]) && finitef(ns[1]) && finitef(ns[2]);
+                                if (!fin3) plast_ok = false;
+                                if (plast_ok && rheo == RHEO_SNOW) {
+                                    const float tc = mrow[2], tsv = mrow[3];
+#pragma unroll
+                                    for (int k = 0; k < 3; ++k) ns[k] = clampf(d.s[k], 1.0f - tc, 1.0f + tsv);
+                                    float det_old = fmaxf(d.s[0] * d.s[1] * d.s[2], 1.0e-6f);
+                                    float det_new = fmaxf(ns[0] * ns[1] * ns[2], 1.0e-6f);
+                                    jp_replace = true;
+                                    jp_new = jp + (__logf(det_old) - __logf(det_new));
+                                    if (!finitef(jp_new)) plast_ok = false;
+                                    jp_new = clampf(jp_new, -5.0f, 5.0f);
+                                } else if (plast_ok && rheo == RHEO_VISCOUS) {
+                                    jp_replace = true;
+                                    jp_new = 0.0f;
+                                } else if (plast_ok) {
+                                    // sand (Drucker-Prager), mpm.wgsl:1421-1457
+                                    const float e0 = __logf(fmaxf(d.s[0], 1.0e-6f)),
+                                                e1 = __logf(fmaxf(d.s[1], 1.0e-6f)),
+                                                e2 = __logf(fmaxf(d.s[2], 1.0e-6f));
+                                    float tre = e0 + e1 + e2, mean = tre / 3.0f;
+                                    float d0 = e0 - mean, d1 = e1 - mean, d2 = e2 - mean;
+                                    float dn = sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+                                    float kappa = (3.0f * lam + 2.0f * mu) / (2.0f * mu);
+                                    if (tre > 0.0f) {
+                                        ns[0] = ns[1] = ns[2] = 1.0f;
+                                        dq = sqrtf(e0 * e0 + e1 * e1 + e2 * e2);
+                                    } else if (dn != 0.0f) {
+                                        float fy = dn + kappa * tre * mrow[1];
+                                        if (fy > 0.0f) {
+                                            float rc = fmaxf(0.0f, -mrow[1] * kappa * tre);
+                                            float sc2 = rc / dn;
+                                            ns[0] = __expf(d0 * sc2 + mean);
+                                            ns[1] = __expf(d1 * sc2 + mean);
+                                            ns[2] = __expf(d2 * sc2 + mean);
+                                            dq = fy;
+                                        }
+                                    }
+                                }
+                                if (plast_ok && (!finitef(ns[0]) || !finitef(ns[1]) || !finitef(ns[2]) ||
+                                                 !finitef(dq) || !finitef(jp + dq)))
+                                    plast_ok = false;
+                                if (plast_ok) {
+                                    float ss[3];
+                                    bool same = true;
+#pragma unroll
+                                    for (int k = 0; k < 3; ++k) {
+                                        ss[k] = clampf(ns[k], 0.05f, 20.0f);
+                                        same = same && (ss[k] == d.s[k]);
+                                    }
+                                    if (!same) {
+                                        udvt(d.U, ss, d.V, Fn);
+                                        d.s[0] = ss[0];
+                                        d.s[1] = ss[1];
+                                        d.s[2] = ss[2];
+                                    }
+                                    jp_new = jp_replace ? jp_new : clampf(jp + dq, 0.0f, 5.0f);
+                                    have_svd = true;
+                                }
+                            }
+                            if (!plast_ok) latch(a.status, FAULT_NUMERICAL);
+                        }
+                        // ---- write the particle state ----
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) ps[(size_t)(CX + k) * cap + slot] = nx[k];
+                        if (a.write_vc || MODE == MODE_G2P) {
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) ps[(size_t)(CV + k) * cap + slot] = vn[k];
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) ps[(size_t)(CC + k) * cap + slot] = Cn[k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) ps[(size_t)(CF + k) * cap + slot] = Fn[k];
+                        if (MODEL == MODEL_SHADER && plast_ok) ps[(size_t)CJP * cap + slot] = jp_new;
+                        if (MODE == MODE_G2P2G && plast_ok) {
+                            if (MODEL == MODEL_CPUDIFF) svd3<MODEL_CPUDIFF>(Fn, d);
+                            (void)have_svd;
+                            affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp_new, dt, vol, m, Cn, sp.A);
+#pragma unroll
+                            for (int k = 0; k < 3; ++k) {
+                                x[k] = nx[k];
+                                v[k] = vn[k];
+                            }
+                        } else {
+                            ok = false;
+                        }
+                    }
+                }
+            }
+            if (ok && MODE != MODE_G2P) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float xp = (x[k] - sc.origin[k]) * sc.inv_dx;
+                    float bf = floorf(xp - 0.5f);
+                    sp.b[k] = int(bf) - (MODEL == MODEL_SHADER ? sc.window[k] : 0);
+                    sp.fx[k] = xp - bf;
+                    sp.mv[k] = m * v[k];
+                }
+                sp.m = m;
+                sp.on = true;
+            }
+        }
+        if (MODE != MODE_G2P) block_scatter(a, sc, sp, s_acc, s_red, s_box);
+    }
+}
+
+hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
+                                hipStream_t s) {
+    (void)ppt;
+    if (a.n_chunks == 0) return hipSuccess;
+    if (blocks <= 0 || uint32_t(blocks) > a.n_chunks) blocks = int(a.n_chunks);
+    dim3 g(blocks), b(BLOCK);
+#define MPM_LAUNCH(M, D) hipLaunchKernelGGL((k_particles<M, D>), g, b, 0, s, a)
+    if (model == MODEL_SHADER) {
+        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_SHADER);
+        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_SHADER);
+        else MPM_LAUNCH(MODE_G2P2G, MODEL_SHADER);
+    } else {
+        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_CPUDIFF);
+        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_CPUDIFF);
+        else MPM_LAUNCH(MODE_G2P2G, MODEL_CPUDIFF);
+    }
+#undef MPM_LAUNCH
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Grid update over the active tiles (mpm.wgsl:1010-1058; CPUDIFF model:
+// mpm_cpu_diff.rs:513-542).  One wave per tile, one lane per cell.
+__global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ tile_scene,
+                                                     float4* __restrict__ gmp, float4* __restrict__ gv,
+                                                     uint32_t* __restrict__ tile_flag,
+                                                     const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ count,
+                                                     uint32_t* __restrict__ next_count, float dt,
+                                                     int zero_after) {
+    const uint32_t n = *count;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
+        const uint32_t t = list[i];
+        const SceneDev sc = scenes[tile_scene[t]];
+        const uint32_t tl = t - sc.tile_base;
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        const int ix = tx * 4 + (lane & 3), iy = ty * 4 + ((lane >> 2) & 3), iz = tz * 4 + (lane >> 4);
+        const uint32_t c = (t << 6) | uint32_t(lane);
+        const float4 q = gmp[c];
+        const float m = q.x;
+        float4 out = make_float4(0.f, 0.f, 0.f, m);
+        const int n_ = int(sc.G);
+        if (sc.model == MODEL_SHADER) {
+            if (m > 1.0e-9f && m == m) {
+                float vx = q.y / m, vy = q.z / m, vz = q.w / m;
+                vy = vy + dt * sc.gravity_y;
+                const float bf = sc.boundary_friction;
+                const int pad = 2;
+                if (ix < pad) { vx = fmaxf(vx, 0.f); vy *= bf; vz *= bf; }
+                if (ix >= n_ - pad) { vx = fminf(vx, 0.f); vy *= bf; vz *= bf; }
+                if (iy < pad) { vy = fmaxf(vy, 0.f); vx *= bf; vz *= bf; }
+                if (iy >= n_ - pad) { vy = fminf(vy, 0.f); vx *= bf; vz *= bf; }
+                if (iz < pad) { vz = fmaxf(vz, 0.f); vx *= bf; vy *= bf; }
+                if (iz >= n_ - pad) { vz = fminf(vz, 0.f); vx *= bf; vy *= bf; }
+                out = make_float4(vx, vy, vz, m);
+            }
+        } else {
+            if (m > 0.0f) {
+                float vx = q.y / m, vy = q.z / m, vz = q.w / m;
+                vy += sc.gravity_y * dt;
+                float node_y = sc.origin[1] + float(iy) * sc.dx;
+                if (node_y < sc.floor_y) {
+                    if (vy < 0.f) vy = 0.f;
+                    float damp = 1.0f - sc.floor_friction;
+                    vx *= damp;
+                    vz *= damp;
+                }
+                out = make_float4(vx, vy, vz, m);
+            }
+        }
+        gv[c] = out;
+        if (zero_after) {
+            gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane == 0) tile_flag[t] = 0u;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && next_count) *next_count = 0u;
+}
+
+hipError_t launch_grid_update(const SceneDev* scenes, const uint32_t* tile_scene, float4* gmp,
+                              float4* gv, uint32_t* tile_flag, const uint32_t* list,
+                              const uint32_t* count, uint32_t* next_count, uint32_t* status,
+                              float dt, int zero_after, int count_gather, int blocks,
+                              hipStream_t s) {
+    (void)status;
+    (void)count_gather;
+    hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, gmp, gv,
+                       tile_flag, list, count, next_count, dt, zero_after);
+    return hipGetLastError();
+}
+
+// Zero the accumulation grid and the flags of the listed tiles; reset count.
+__global__ __launch_bounds__(256) void k_clear_tiles(float4* __restrict__ gmp, uint32_t* __restrict__ tile_flag,
+                                                     const uint32_t* __restrict__ list,
+                                                     uint32_t* __restrict__ count) {
+    const uint32_t n = *count;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
+        const uint32_t t = list[i];
+        gmp[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 0) tile_flag[t] = 0u;
+    }
+}
+
+// Resets the counter after every block has read it (separate tiny launch).
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+hipError_t launch_clear_tiles(float4* gmp, uint32_t* tile_flag, const uint32_t* list,
+                              uint32_t* count, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, gmp, tile_flag, list, count);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, count, 0u);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Tile keys (shaders/mpm_particle_order.wgsl:13-25): cell = floor((x - o) *
+// inv_dx) - window, clamped to [0, G-1], /4, linearised x-fastest; the global
+// key adds the scene's tile base.  Written at the particle's original id so
+// that a stable sort over ids 0..n-1 reproduces the reference order.
+__global__ __launch_bounds__(256) void k_tile_keys(StepArgs a, uint32_t* __restrict__ keys_by_id) {
+    for (uint32_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+        const Chunk ch = a.chunks[ci];
+        if (threadIdx.x >= ch.count) continue;
+        const SceneDev& sc = a.scenes[ch.scene];
+        const uint32_t slot = ch.start + threadIdx.x;
+        uint32_t t[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float c = floorf((a.ps[(size_t)(CX + k) * a.cap + slot] - sc.origin[k]) * sc.inv_dx);
+            int ci2 = int(c) - sc.window[k];
+            ci2 = min(max(ci2, 0), int(sc.G) - 1);
+            t[k] = uint32_t(ci2) >> 2;
+        }
+        const uint32_t id = __float_as_uint(a.ps[(size_t)CID * a.cap + slot]);
+        keys_by_id[id] = sc.tile_base + t[0] + sc.W * (t[1] + sc.W * t[2]);
+    }
+}
+
+hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t s) {
+    if (a.n_chunks == 0) return hipSuccess;
+    unsigned blocks = a.n_chunks < 65535u ? a.n_chunks : 65535u;
+    hipLaunchKernelGGL(k_tile_keys, dim3(blocks), dim3(256), 0, s, a, keys_by_id);
+    return hipGetLastError();
+}
+
+// dst slot j <- src slot slot_of_id[sorted_ids[j]], all planes.
+__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ src, float* __restrict__ dst,
+                                                 uint32_t cap, const uint32_t* __restrict__ sorted_ids,
+                                                 const uint32_t* __restrict__ slot_of_id, uint32_t n) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t old = slot_of_id[sorted_ids[j]];
+#pragma unroll 6
+        for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * cap + j] = src[(size_t)k * cap + old];
+    }
+}
+
+hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* sorted_ids,
+                          const uint32_t* slot_of_id, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, src, dst, cap, sorted_ids, slot_of_id, n);
+    return hipGetLastError();
+}
+
+__global__ void k_index_slots(const float* __restrict__ ps, uint32_t cap, uint32_t* __restrict__ slot_of_id,
+                              uint32_t n) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        slot_of_id[__float_as_uint(ps[(size_t)CID * cap + j])] = j;
+}
+
+hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,
+                              hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_index_slots, dim3(blocks), dim3(256), 0, s, ps, cap, slot_of_id, n);
+    return hipGetLastError();
+}
+
+__global__ void k_unpermute(const float* __restrict__ ps, uint32_t cap, int comp0, int ncomp,
+                            const uint32_t* __restrict__ slot_of_id, float* __restrict__ out, uint32_t n) {
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
+        const uint32_t s = slot_of_id[id];
+        for (int c = 0; c < ncomp; ++c) out[(size_t)id * ncomp + c] = ps[(size_t)(comp0 + c) * cap + s];
+    }
+}
+
+hipError_t launch_unpermute(const float* ps, uint32_t cap, int comp0, int ncomp,
+                            const uint32_t* slot_of_id, float* out, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_unpermute, dim3(blocks), dim3(256), 0, s, ps, cap, comp0, ncomp, slot_of_id, out, n);
+    return hipGetLastError();
+}
+
+__global__ void k_permute_in(float* __restrict__ ps, uint32_t cap, int comp0, int ncomp,
+                             const uint32_t* __restrict__ slot_of_id, const float* __restrict__ in, uint32_t n) {
+    for (uint32_t id = blockIdx.x * blockDim.x + threadIdx.x; id < n; id += gridDim.x * blockDim.x) {
+        const uint32_t s = slot_of_id[id];
+        for (int c = 0; c < ncomp; ++c) ps[(size_t)(comp0 + c) * cap + s] = in[(size_t)id * ncomp + c];
+    }
+}
+
+hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
+                             const uint32_t* slot_of_id, const float* in, uint32_t n,
+                             hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_permute_in, dim3(blocks), dim3(256), 0, s, ps, cap, comp0, ncomp, slot_of_id, in, n);
+    return hipGetLastError();
+}
+
+__global__ void k_grid_to_dense(const float4* __restrict__ gmp, const SceneDev* __restrict__ scenes,
+                                uint32_t scene, uint32_t G, float* __restrict__ mass, float* __restrict__ mom) {
+    const SceneDev sc = scenes[scene];
+    const size_t total = (size_t)G * G * G;
+    for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < total; c += (size_t)gridDim.x * blockDim.x) {
+        int ix = int(c % G), iy = int((c / G) % G), iz = int(c / ((size_t)G * G));
+        float4 q = gmp[cell_index(sc, ix, iy, iz)];
+        mass[c] = q.x;
+        mom[3 * c] = q.y;
+        mom[3 * c + 1] = q.z;
+        mom[3 * c + 2] = q.w;
+    }
+}
+
+hipError_t launch_grid_to_dense(const float4* gmp, const SceneDev* scenes, uint32_t scene,
+                                uint32_t G, float* mass, float* mom, hipStream_t s) {
+    size_t total = (size_t)G * G * G;
+    unsigned blocks = (unsigned)((total + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_grid_to_dense, dim3(blocks), dim3(256), 0, s, gmp, scenes, scene, G, mass, mom);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic gather-order P2G (mpm_deterministic_p2g.wgsl:12-150).
+__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ tb,
+                              uint32_t* __restrict__ te) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t k = keys[j];
+        if (j == 0 || keys[j - 1] != k) tb[k] = j;
+        if (j + 1 == n || keys[j + 1] != k) te[k] = j + 1;
+    }
+}
+
+hipError_t launch_tile_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_begin,
+                              uint32_t* tile_end, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_tile_ranges, dim3(blocks), dim3(256), 0, s, sorted_keys, n, tile_begin, tile_end);
+    return hipGetLastError();
+}
+
+// Every tile within one tile of an occupied tile may receive mass.
+__global__ void k_mark_dilated(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                               const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flag,
+                               uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t t = keys[j];
+        if (j != 0 && keys[j - 1] == t) continue;
+        const SceneDev sc = scenes[tile_scene[t]];
+        const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dxx = -1; dxx <= 1; ++dxx) {
+                    int nx = tx + dxx, ny = ty + dy, nz = tz + dz;
+                    if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
+                    mark_tile(flag, list, count, sc.tile_base + uint32_t(nx + W * (ny + W * nz)));
+                }
+    }
+}
+
+hipError_t launch_mark_dilated(const SceneDev* scenes, const uint32_t* tile_scene,
+                               const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_flag,
+                               uint32_t* list, uint32_t* count, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_mark_dilated, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, sorted_keys, n,
+                       tile_flag, list, count);
+    return hipGetLastError();
+}
+
+// Per particle: affine matrix A (planes 0-8) and m v (planes 9-11).
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_gather_prepare(StepArgs a, float* __restrict__ aff) {
+    const uint32_t cap = a.cap;
+    const float* ps = a.ps;
+    for (uint32_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+        const Chunk ch = a.chunks[ci];
+        if (threadIdx.x >= ch.count) continue;
+        const SceneDev sc = a.scenes[ch.scene];
+        const uint32_t slot = ch.start + threadIdx.x;
+        float F[9], C[9];
+        for (int k = 0; k < 9; ++k) {
+            F[k] = ps[(size_t)(CF + k) * cap + slot];
+            C[k] = ps[(size_t)(CC + k) * cap + slot];
+        }
+        const float m = ps[(size_t)CMASS * cap + slot], vol = ps[(size_t)CVOL * cap + slot];
+        const float jp = ps[(size_t)CJP * cap + slot];
+        const float* mrow = mat_row(a, sc, __float_as_uint(ps[(size_t)CMAT * cap + slot]));
+        float mu = sc.mu, lam = sc.lam;
+        if (a.has_E) {
+            float E = ps[(size_t)CE * cap + slot];
+            mu = E / (2.0f * (1.0f + sc.poisson));
+            lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
+        }
+        Svd32 d;
+        svd3<MODEL>(F, d);
+        float A[9];
+        affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp, a.dt, vol, m, C, A);
+        for (int k = 0; k < 9; ++k) aff[(size_t)k * cap + slot] = A[k];
+        for (int k = 0; k < 3; ++k) aff[(size_t)(9 + k) * cap + slot] = m * ps[(size_t)(CV + k) * cap + slot];
+    }
+}
+
+hipError_t launch_gather_prepare(const StepArgs& a, float* affine, int blocks, hipStream_t s) {
+    if (a.n_chunks == 0) return hipSuccess;
+    if (blocks <= 0 || uint32_t(blocks) > a.n_chunks) blocks = int(a.n_chunks);
+    // the model is uniform per context; scenes[0] tells which
+    SceneDev h;
+    (void)h;
+    hipLaunchKernelGGL((k_gather_prepare<MODEL_SHADER>), dim3(blocks), dim3(256), 0, s, a, affine);
+    return hipGetLastError();
+}
+
+// One wave per active tile, one lane per cell.  For each cell, particles of
+// the 27 neighbour tiles are summed in tile order dx + 3 dy + 9 dz, then in
+// ascending sorted slot (mpm_deterministic_p2g.wgsl:100-144): a fixed order,
+// so the result is bitwise repeatable.
+__global__ __launch_bounds__(64) void k_gather_p2g(StepArgs a, const uint32_t* __restrict__ tile_scene,
+                                                   const uint32_t* __restrict__ tb,
+                                                   const uint32_t* __restrict__ te,
+                                                   const float* __restrict__ aff,
+                                                   const uint32_t* __restrict__ list,
+                                                   const uint32_t* __restrict__ count) {
+    const uint32_t n = *count;
+    const int lane = threadIdx.x;
+    const uint32_t cap = a.cap;
+    for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+        const uint32_t t = list[i];
+        const SceneDev sc = a.scenes[tile_scene[t]];
+        const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        const int cx = tx * 4 + (lane & 3), cy = ty * 4 + ((lane >> 2) & 3), cz = tz * 4 + (lane >> 4);
+        float ms = 0.f, p0 = 0.f, p1 = 0.f, p2 = 0.f;
+        for (int ti = 0; ti < 27; ++ti) {
+            const int nx = tx + ti % 3 - 1, ny = ty + (ti / 3) % 3 - 1, nz = tz + ti / 9 - 1;
+            if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
+            const uint32_t key = sc.tile_base + uint32_t(nx + W * (ny + W * nz));
+            const uint32_t jb = tb[key], je = te[key];
+            for (uint32_t j = jb; j < je; ++j) {
+                float xp[3], fx[3];
+                int off[3];
+                bool in = true;
+                for (int k = 0; k < 3; ++k) {
+                    xp[k] = (a.ps[(size_t)(CX + k) * cap + j] - sc.origin[k]) * sc.inv_dx;
+                    float bf = floorf(xp[k] - 0.5f);
+                    fx[k] = xp[k] - bf;
+                    const int c = k == 0 ? cx : (k == 1 ? cy : cz);
+                    off[k] = c - (int(bf) - sc.window[k]);
+                    in = in && off[k] >= 0 && off[k] <= 2;
+                }
+                if (!in) continue;
+                float w0[3], w1[3], w2[3];
+                quad_weights(fx[0], w0);
+                quad_weights(fx[1], w1);
+                quad_weights(fx[2], w2);
+                const float wt = w0[off[0]] * w1[off[1]] * w2[off[2]];
+                const float d0 = (float(off[0]) - fx[0]) * sc.dx, d1 = (float(off[1]) - fx[1]) * sc.dx,
+                            d2 = (float(off[2]) - fx[2]) * sc.dx;
+                float A[9];
+                for (int k = 0; k < 9; ++k) A[k] = aff[(size_t)k * cap + j];
+                const float m = a.ps[(size_t)CMASS * cap + j];
+                ms += wt * m;
+                p0 += wt * (aff[(size_t)9 * cap + j] + A[0] * d0 + A[1] * d1 + A[2] * d2);
+                p1 += wt * (aff[(size_t)10 * cap + j] + A[3] * d0 + A[4] * d1 + A[5] * d2);
+                p2 += wt * (aff[(size_t)11 * cap + j] + A[6] * d0 + A[7] * d1 + A[8] * d2);
+            }
+        }
+        const bool valid = cx < int(sc.G) && cy < int(sc.G) && cz < int(sc.G);
+        a.gmp[(t << 6) | uint32_t(lane)] = valid ? make_float4(ms, p0, p1, p2) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene,
+                             const uint32_t* tile_begin, const uint32_t* tile_end,
+                             const float* affine, const uint32_t* list, const uint32_t* count,
+                             int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_p2g, dim3(blocks), dim3(64), 0, s, a, tile_scene, tile_begin, tile_end,
+                       affine, list, count);
+    return hipGetLastError();
+}
+
+}  // namespace mpm

@@ -1,0 +1,376 @@
+"""Host-side mirror of molla's ``MpmSimulation`` over the C ABI.
+
+Method names, argument meaning and error behaviour follow
+``crates/molla-solvers/src/mpm.rs`` (``new`` :834, ``set_particles`` :1451,
+``set_particles_with_velocity`` :1458-1519, ``step``/``step_n`` :2232-2272,
+``read_positions_with`` :2718, ``read_velocities_with`` :2760,
+``read_grid_mass_momentum`` :2792, ``read_active_grid_status_with`` :1418,
+``set_p2g_mode`` :1642, ``set_materials`` :1570) so that tests read like the
+reference's own.  Errors surface as ``MpmError`` carrying the library's
+message (the reference returns ``Error::Solver`` / ``Error::Gpu``).
+
+The compute runs in ``libmolla_mpm.so`` (HIP, gfx950).  There is no CPU
+fallback: if the library is missing or no device is present, construction
+raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Optional, Sequence
+
+import numpy as np
+
+from .config import MODEL_SHADER, MpmConfig, material_row
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmolla_mpm.so")
+
+MAX_MATERIALS = 8
+
+P2G_FUSED = 0
+P2G_GLOBAL_ATOMIC = 1
+P2G_DETERMINISTIC = 2
+_P2G_NAMES = {"fused": P2G_FUSED, "two_pass": P2G_FUSED, "global_atomic": P2G_GLOBAL_ATOMIC,
+              "legacy": P2G_GLOBAL_ATOMIC, "deterministic": P2G_DETERMINISTIC}
+
+FIELDS = {"position": (0, 3), "velocity": (1, 3), "C": (2, 9), "F": (3, 9), "jp": (4, 1),
+          "mass": (5, 1), "volume": (6, 1), "youngs": (7, 1), "material": (8, 1)}
+
+STAGES = ["sort", "p2g", "grid", "g2p2g", "g2p"]
+
+
+class MpmError(RuntimeError):
+    """Raised when a library call fails (status code + last-error message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Material(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("rheology", "dp_alpha", "theta_c", "theta_s", "hardening",
+                                         "velocity_damping", "affine_damping", "bulk")]
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [
+        ("grid_size", C.c_uint32),
+        ("cell_size", C.c_float),
+        ("grid_origin", C.c_float * 3),
+        ("grid_window_cells", C.c_int32 * 3),
+        ("gravity_y", C.c_float),
+        ("youngs", C.c_float),
+        ("poisson", C.c_float),
+        ("boundary_friction", C.c_float),
+        ("particle_mass", C.c_float),
+        ("particle_volume", C.c_float),
+        ("max_particles", C.c_uint32),
+        ("model", C.c_uint32),
+        ("floor_y", C.c_float),
+        ("floor_friction", C.c_float),
+        ("n_materials", C.c_uint32),
+        ("materials", Material * MAX_MATERIALS),
+    ]
+
+
+class StatusInfo(C.Structure):
+    _fields_ = [("fault_bits", C.c_uint32), ("completed_gathers", C.c_uint32),
+                ("active_tiles", C.c_uint32), ("sorts", C.c_uint32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "mpm_abi_version": (C.c_int, []),
+    "mpm_create": (C.c_int, [C.POINTER(SceneConfig), C.POINTER(C.c_void_p)]),
+    "mpm_create_batch": (C.c_int, [C.POINTER(SceneConfig), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mpm_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_last_error": (C.c_char_p, [C.c_void_p]),
+    "mpm_set_particles": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mpm_set_particles_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mpm_write_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    "mpm_read_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    "mpm_read_positions": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mpm_read_velocities": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mpm_set_materials": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Material), C.c_uint32]),
+    "mpm_set_scene_params": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SceneConfig)]),
+    "mpm_set_p2g_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_set_sort_interval": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mpm_set_write_vc": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "mpm_step_n": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32]),
+    "mpm_step_n_graph": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32]),
+    "mpm_synchronize": (C.c_int, [C.c_void_p]),
+    "mpm_read_grid_mass_momentum": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mpm_read_status": (C.c_int, [C.c_void_p, C.POINTER(StatusInfo)]),
+    "mpm_reset_status": (C.c_int, [C.c_void_p]),
+    "mpm_sort_keys_debug": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mpm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_read_profile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mpm_reset_profile": (C.c_int, [C.c_void_p]),
+    "mpm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]),
+    "mpm_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+}
+
+
+def lib():
+    """Load libmolla_mpm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MpmError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> List[str]:
+    return list(_SIGS)
+
+
+def scene_config(cfg: MpmConfig, max_particles: Optional[int] = None) -> SceneConfig:
+    s = SceneConfig()
+    s.grid_size = int(cfg.grid_size)
+    s.cell_size = float(cfg.cell_size)
+    for a in range(3):
+        s.grid_origin[a] = float(cfg.grid_origin[a])
+        s.grid_window_cells[a] = int(cfg.grid_window_cells[a])
+    s.gravity_y = float(cfg.gravity_y)
+    s.youngs = float(cfg.youngs)
+    s.poisson = float(cfg.poisson)
+    s.boundary_friction = float(cfg.boundary_friction)
+    s.particle_mass = float(cfg.particle_mass)
+    s.particle_volume = float(cfg.particle_volume)
+    s.max_particles = int(cfg.max_particles if max_particles is None else max_particles)
+    s.model = int(cfg.model)
+    s.floor_y = float(cfg.floor_y)
+    s.floor_friction = float(cfg.floor_friction)
+    rows = cfg.material_rows()
+    s.n_materials = len(rows)
+    for i, r in enumerate(rows):
+        for name, val in zip(Material._fields_, r):
+            setattr(s.materials[i], name[0], float(val))
+    return s
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class MpmSimulation:
+    """One MPM simulation (or a batch of independent scenes) on the GPU."""
+
+    def __init__(self, cfg, max_particles: Optional[int] = None):
+        cfgs = list(cfg) if isinstance(cfg, (list, tuple)) else [cfg]
+        self.cfgs: List[MpmConfig] = cfgs
+        self.cfg = cfgs[0]
+        arr = (SceneConfig * len(cfgs))(*[scene_config(c, max_particles) for c in cfgs])
+        h = C.c_void_p()
+        L = lib()
+        rc = L.mpm_create_batch(arr, len(cfgs), C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = L.mpm_last_error(h).decode() if h.value else "create failed"
+            if h.value:
+                L.mpm_destroy(h)
+            self._h = None
+            raise MpmError(rc, msg)
+        self.counts = [0] * len(cfgs)
+
+    # -- construction helpers mirroring the reference names --
+    @classmethod
+    def new(cls, cfg: MpmConfig) -> "MpmSimulation":
+        return cls(cfg)
+
+    @classmethod
+    def batch(cls, cfgs: Sequence[MpmConfig]) -> "MpmSimulation":
+        return cls(list(cfgs))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().mpm_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int):
+        if rc != 0:
+            raise MpmError(rc, lib().mpm_last_error(self._h).decode())
+
+    @property
+    def n(self) -> int:
+        return int(sum(self.counts))
+
+    # -- particles --
+    def set_particles(self, positions, velocities=None):
+        """``set_particles`` / ``set_particles_with_velocity`` (mpm.rs:1451-1519)."""
+        pos = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+        vel = None
+        if velocities is not None and len(velocities) > 0:
+            vel = np.ascontiguousarray(np.asarray(velocities, np.float32).reshape(-1, 3))
+            if vel.shape != pos.shape:
+                raise MpmError(1, f"velocities len ({len(vel)}) != positions len ({len(pos)})")
+        if len(self.cfgs) != 1:
+            raise MpmError(1, "use set_particles_batch on a batch")
+        self._check(lib().mpm_set_particles(self._h, _ptr(pos), _ptr(vel), pos.shape[0]))
+        self.counts = [pos.shape[0]]
+
+    set_particles_with_velocity = set_particles
+
+    def set_particles_batch(self, positions: Sequence[np.ndarray], velocities=None):
+        pos = [np.asarray(p, np.float32).reshape(-1, 3) for p in positions]
+        counts = np.asarray([p.shape[0] for p in pos], np.uint32)
+        allp = np.ascontiguousarray(np.concatenate(pos) if pos else np.zeros((0, 3), np.float32))
+        allv = None
+        if velocities is not None:
+            allv = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float32).reshape(-1, 3)
+                                                        for v in velocities]))
+        self._check(lib().mpm_set_particles_batch(self._h, _ptr(counts), _ptr(allp), _ptr(allv)))
+        self.counts = counts.tolist()
+
+    def write_field(self, name: str, data):
+        fid, nc = FIELDS[name]
+        dt = np.uint32 if name == "material" else np.float32
+        a = np.ascontiguousarray(np.asarray(data, dt).reshape(self.n, nc) if nc > 1
+                                 else np.asarray(data, dt).reshape(self.n))
+        self._check(lib().mpm_write_field(self._h, fid, _ptr(a), self.n))
+
+    def read_field(self, name: str) -> np.ndarray:
+        fid, nc = FIELDS[name]
+        dt = np.uint32 if name == "material" else np.float32
+        out = np.zeros((self.n, nc) if nc > 1 else (self.n,), dt)
+        self._check(lib().mpm_read_field(self._h, fid, _ptr(out), self.n))
+        if nc == 9:
+            return out.reshape(self.n, 3, 3)
+        return out
+
+    def read_positions(self) -> np.ndarray:
+        return self.read_field("position")
+
+    def read_velocities(self) -> np.ndarray:
+        return self.read_field("velocity")
+
+    read_positions_with = read_positions
+    read_velocities_with = read_velocities
+
+    # -- materials / parameters --
+    def set_materials(self, rheologies: Iterable, scene: int = 0):
+        rows = [material_row(r) for r in rheologies]
+        arr = (Material * len(rows))(*[Material(*[float(t) for t in r]) for r in rows])
+        self._check(lib().mpm_set_materials(self._h, scene, arr, len(rows)))
+
+    def set_particle_materials(self, ids):
+        self.write_field("material", ids)
+
+    def set_scene_params(self, cfg: MpmConfig, scene: int = 0):
+        sc = scene_config(cfg, self.cfgs[scene].max_particles)
+        self._check(lib().mpm_set_scene_params(self._h, scene, C.byref(sc)))
+        self.cfgs[scene] = cfg
+
+    def set_p2g_mode(self, mode):
+        if isinstance(mode, bool):  # reference: set_p2g_mode(two_pass: bool)
+            mode = P2G_FUSED if mode else P2G_GLOBAL_ATOMIC
+        if isinstance(mode, str):
+            mode = _P2G_NAMES[mode]
+        self._check(lib().mpm_set_p2g_mode(self._h, int(mode)))
+
+    def set_deterministic_p2g(self, enabled: bool):
+        self.set_p2g_mode(P2G_DETERMINISTIC if enabled else P2G_FUSED)
+
+    def set_sort_interval(self, every: int):
+        self._check(lib().mpm_set_sort_interval(self._h, int(every)))
+
+    def set_write_vc(self, every_substep: bool):
+        self._check(lib().mpm_set_write_vc(self._h, 1 if every_substep else 0))
+
+    def set_launch(self, particle_blocks: int = 0, grid_blocks: int = 0):
+        self._check(lib().mpm_set_launch(self._h, int(particle_blocks), int(grid_blocks)))
+
+    # -- stepping --
+    def step(self, dt: float):
+        self.step_n(dt, 1)
+
+    def step_n(self, dt: float, substeps: int):
+        self._check(lib().mpm_step_n(self._h, float(dt), int(substeps)))
+
+    def step_n_graph(self, dt: float, substeps: int):
+        self._check(lib().mpm_step_n_graph(self._h, float(dt), int(substeps)))
+
+    def synchronize(self):
+        self._check(lib().mpm_synchronize(self._h))
+
+    # -- reads --
+    def read_grid_mass_momentum(self, scene: int = 0):
+        G = int(self.cfgs[scene].grid_size)
+        m = np.zeros(G ** 3, np.float32)
+        p = np.zeros((G ** 3, 3), np.float32)
+        self._check(lib().mpm_read_grid_mass_momentum(self._h, scene, _ptr(m), _ptr(p)))
+        return m, p
+
+    def read_status(self) -> dict:
+        st = StatusInfo()
+        self._check(lib().mpm_read_status(self._h, C.byref(st)))
+        return {"fault_bits": st.fault_bits, "overflow": bool(st.fault_bits & 1),
+                "numerical_fault": bool(st.fault_bits & 2), "completed_gathers": st.completed_gathers,
+                "active_tiles": st.active_tiles, "sorts": st.sorts,
+                "faulted": st.fault_bits != 0}
+
+    read_active_grid_status = read_status
+
+    def reset_status(self):
+        self._check(lib().mpm_reset_status(self._h))
+
+    def sort_keys_debug(self, scene: int = 0, capacity: Optional[int] = None):
+        cnt = self.counts[scene]
+        cap = cnt if capacity is None else int(capacity)
+        keys = np.zeros(cap, np.uint32)
+        ids = np.zeros(cap, np.uint32)
+        self._check(lib().mpm_sort_keys_debug(self._h, scene, cap, _ptr(keys), _ptr(ids)))
+        return keys, ids
+
+    # -- profiling --
+    def set_profiling(self, enabled: bool):
+        self._check(lib().mpm_set_profiling(self._h, 1 if enabled else 0))
+
+    def read_profile(self) -> dict:
+        ms = np.zeros(len(STAGES), np.float32)
+        n = np.zeros(len(STAGES), np.uint32)
+        self._check(lib().mpm_read_profile(self._h, _ptr(ms), _ptr(n)))
+        return {s: (float(ms[i]), int(n[i])) for i, s in enumerate(STAGES)}
+
+    def reset_profile(self):
+        self._check(lib().mpm_reset_profile(self._h))
+
+    # -- differentiable path --
+    def diff_forward(self, dt: float, steps: int, checkpoint_every: int = 16):
+        self._check(lib().mpm_diff_forward(self._h, float(dt), int(steps), int(checkpoint_every)))
+
+    def diff_backward(self, grad_x_final, grad_v_final=None):
+        n = self.n
+        gx = np.ascontiguousarray(np.asarray(grad_x_final, np.float32).reshape(n, 3))
+        gv = None if grad_v_final is None else np.ascontiguousarray(
+            np.asarray(grad_v_final, np.float32).reshape(n, 3))
+        ox = np.zeros((n, 3), np.float32)
+        ov = np.zeros((n, 3), np.float32)
+        oe = np.zeros(n, np.float32)
+        ml = np.zeros(2, np.float64)
+        self._check(lib().mpm_diff_backward(self._h, _ptr(gx), _ptr(gv), _ptr(ox), _ptr(ov),
+                                            _ptr(oe), _ptr(ml)))
+        return {"x": ox, "v": ov, "E": oe, "mu": float(ml[0]), "lambda": float(ml[1])}

@@ -1,0 +1,261 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer work (tile keys, sorted ids) must be bit-exact.  Floating-point
+state is compared with the FP64 oracle; the tolerance of each comparison is
+stated where it is asserted.  For whole-step comparisons the bound is
+``max(8 * e32, floor)`` where ``e32`` is the error of the oracle's own f32
+restatement against FP64 on the same input: the HIP path must be as accurate
+as a plain f32 implementation of the reference arithmetic.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from molla_amd import Fluid, MpmConfig, MpmSimulation, Sand, Snow, Viscous, workloads
+from tests.fixtures import fill_box, order_config, points
+from tests.gpu_util import log, oracle_state, rel_err, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+
+# ---------------------------------------------------------------- ordering
+@pytest.mark.parametrize("size", [35, 1536])
+def test_sort_keys_and_ids_bit_exact(size):
+    # src/mpm_particle_order_tests.rs:66-150 with count in [513, 257, 1, 0, 63]
+    cfg = order_config(size)
+    for count in [513, 257, 1, 63]:
+        pos = points(count)
+        if count > 1:
+            pos[0] = [-1.0625] * 3
+            pos[1] = [size * cfg.cell_size] * 3
+        log(f"sort keys size={size} count={count}")
+        sim = MpmSimulation(cfg, max_particles=513)
+        sim.set_particles(pos)
+        keys, ids = sim.sort_keys_debug(capacity=513)
+        ek, ei = orc.sorted_tile_keys(cfg, pos, capacity=513)
+        assert np.array_equal(keys, ek)
+        assert np.array_equal(ids, ei)
+        sim.close()
+
+
+def test_sort_keys_large_random_bit_exact():
+    cfg = MpmConfig.sand().copy(grid_size=256, cell_size=1 / 256, grid_origin=(0, 0, 0))
+    rng = np.random.default_rng(5)
+    x = rng.random((300_000, 3)).astype(np.float32)
+    x[:1000] = x[1000:2000]  # ties
+    log("sort keys 300k random")
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x)
+    keys, ids = sim.sort_keys_debug()
+    ek, ei = orc.sorted_tile_keys(cfg, x)
+    assert np.array_equal(keys, ek) and np.array_equal(ids, ei)
+
+
+# ---------------------------------------------------------------- P2G / grid
+def test_grid_mass_matches_particle_mass_and_oracle_p2g():
+    # tests/mpm_gpu.rs:840-885 (rel < 1e-4, |p| < 1e-6), plus cellwise vs the oracle
+    cfg = MpmConfig.sand().copy(gravity_y=0.0, grid_origin=(-1.6, -1.6, -1.6), grid_size=32,
+                                cell_size=0.1, particle_mass=2.5e-3, boundary_friction=1.0)
+    x = fill_box([-0.35] * 3, [0.35] * 3, 7, 19)
+    log("grid mass")
+    sim, _ = run_gpu(cfg, x, dt=1 / 240, substeps=1)
+    m, p = sim.read_grid_mass_momentum()
+    exp = cfg.particle_mass * len(x)
+    assert abs(m.astype(np.float64).sum() - exp) / exp < 1e-4
+    assert np.abs(p).max() < 1e-6
+    gm, gp = orc.p2g(cfg, oracle_state(cfg, x), 1 / 240)
+    assert rel_err(m, gm) < 1e-6
+
+
+def test_p2g_with_stress_matches_oracle():
+    cfg = MpmConfig.sand().copy(youngs=2.0e4, rheology=Viscous(0, 0))
+    rng = np.random.default_rng(7)
+    x = fill_box([-0.2, 0.2, -0.2], [0.2, 0.6, 0.2], 12, 9)
+    n = len(x)
+    F = np.eye(3) + 0.05 * rng.standard_normal((n, 3, 3))
+    v = 0.3 * rng.standard_normal((n, 3))
+    C = 0.5 * rng.standard_normal((n, 3, 3))
+    for mode in ("fused", "global_atomic", "deterministic"):
+        log(f"p2g vs oracle mode={mode}")
+        sim = MpmSimulation(cfg, max_particles=n)
+        sim.set_particles(x, v.astype(np.float32))
+        sim.write_field("F", F.astype(np.float32))
+        sim.write_field("C", C.astype(np.float32))
+        sim.set_p2g_mode(mode)
+        sim.step_n(1e-4, 1)
+        m, p = sim.read_grid_mass_momentum()
+        st = oracle_state(cfg, x, v.astype(np.float32), F=F.astype(np.float32))
+        st.C[:] = C.astype(np.float32)
+        gm, gp = orc.p2g(cfg, st, 1e-4)
+        em, ep = rel_err(m, gm), rel_err(p, gp)
+        log(f"  mass err {em:.2e} momentum err {ep:.2e}")
+        assert em < 1e-6 and ep < 1e-5
+
+
+# ---------------------------------------------------------------- whole steps
+def test_single_particle_free_fall_is_exact():
+    # tests/mpm_freefall_regression_gpu.rs:76-111
+    cfg = MpmConfig.snow()
+    sim = MpmSimulation(cfg, max_particles=1)
+    sim.set_particles(np.array([[0.013, 0.2, 0.017]], np.float32))
+    dt = 1.0 / 480.0
+    for step in range(20):
+        sim.step(dt)
+        pos = sim.read_positions()
+        vel = sim.read_velocities()
+        vy = -9.81 * dt * (step + 1)
+        assert abs(vel[0, 1] - vy) < 1e-4
+        assert abs(vel[0, 0]) < 1e-5 and abs(vel[0, 2]) < 1e-5
+        assert abs(pos[0, 0] - 0.013) < 1e-5 and abs(pos[0, 2] - 0.017) < 1e-5
+
+
+def _compare_with_oracle(cfg, x, dt, substeps, v=None, mode="fused", fields=None, tag=""):
+    log(f"whole-step parity {tag} mode={mode} n={len(x)} substeps={substeps}")
+    _, g = run_gpu(cfg, x, v, dt, substeps, mode=mode, fields=fields)
+    kw = {}
+    if fields:
+        kw = {k: fields[k] for k in ("mass", "volume", "material", "youngs") if k in fields}
+    def ostate(dtype):
+        return oracle_state(cfg, x, v, dtype=dtype, mass=kw.get("mass"), vol=kw.get("volume"),
+                            mat=kw.get("material"), E=kw.get("youngs"))
+    o64 = ostate(np.float64)
+    o32 = ostate(np.float32)
+    orc.step(cfg, o64, dt, substeps)
+    orc.step(cfg, o32, dt, substeps)
+    errs = {}
+    for f, ref, low in (("x", o64.x, o32.x), ("v", o64.v, o32.v), ("F", o64.F, o32.F), ("C", o64.C, o32.C)):
+        e = rel_err(g[f], ref)
+        e32 = rel_err(low, ref)
+        errs[f] = (e, e32)
+    log("  " + " ".join(f"{k}: gpu {e:.2e} f32-oracle {e32:.2e}" for k, (e, e32) in errs.items()))
+    assert g["status"]["fault_bits"] == 0
+    return errs
+
+
+def _assert_errs(errs, floors):
+    for f, (e, e32) in errs.items():
+        assert e <= max(8 * e32, floors[f]), (f, e, e32)
+
+
+def test_elastic_cube_matches_fp64_oracle():
+    cfg, x, meta = workloads.elastic_cube(8192, seed=1)
+    errs = _compare_with_oracle(cfg, x, meta["dt"], 100, tag="elastic_cube")
+    _assert_errs(errs, {"x": 1e-6, "v": 1e-3, "F": 1e-5, "C": 1e-3})
+
+
+@pytest.mark.parametrize("rheo", ["sand", "snow", "viscous", "fluid"])
+def test_rheologies_match_fp64_oracle(rheo):
+    r = {"sand": Sand(), "snow": Snow(), "viscous": Viscous(4.0, 8.0), "fluid": Fluid(2.0e3)}[rheo]
+    cfg = MpmConfig.sand().copy(rheology=r, youngs=2.0e3 if rheo != "snow" else 1.4e4,
+                                poisson=0.3 if rheo != "snow" else 0.2)
+    x = fill_box([-0.15, 0.05, -0.15], [0.15, 0.35, 0.15], 12, 31)
+    errs = _compare_with_oracle(cfg, x, 1 / 960, 60, tag=rheo)
+    _assert_errs(errs, {"x": 1e-5, "v": 5e-3, "F": 1e-4, "C": 5e-3})
+
+
+def test_p2g_modes_agree_and_deterministic_is_bitwise_repeatable():
+    cfg, x, meta = workloads.elastic_cube(8192, seed=2)
+    res = {}
+    for mode in ("fused", "global_atomic", "deterministic"):
+        log(f"mode {mode}")
+        _, res[mode] = run_gpu(cfg, x, None, meta["dt"], 50, mode=mode)
+    for mode in ("global_atomic", "deterministic"):
+        for f in ("x", "v", "F"):
+            e = rel_err(res[mode][f], res["fused"][f])
+            log(f"  {mode} vs fused {f}: {e:.2e}")
+            assert e < {"x": 1e-6, "v": 1e-3, "F": 1e-5}[f]
+    _, again = run_gpu(cfg, x, None, meta["dt"], 50, mode="deterministic")
+    for f in ("x", "v", "C", "F"):
+        assert np.array_equal(again[f], res["deterministic"][f])
+
+
+def test_sort_interval_and_write_vc_do_not_change_results_beyond_rounding():
+    cfg, x, meta = workloads.elastic_cube(8192, seed=3)
+    _, base = run_gpu(cfg, x, None, meta["dt"], 40, sort_every=0)
+    for kw in ({"sort_every": 1}, {"sort_every": 7}, {"write_vc": False}):
+        log(f"variant {kw}")
+        _, r = run_gpu(cfg, x, None, meta["dt"], 40, **kw)
+        for f in ("x", "v", "F"):
+            assert rel_err(r[f], base[f]) < {"x": 1e-6, "v": 1e-3, "F": 1e-5}[f]
+
+
+def test_graph_replays_match_eager():
+    cfg, x, meta = workloads.elastic_cube(4096, seed=4)
+    dt = meta["dt"]
+    eager = MpmSimulation(cfg, max_particles=len(x))
+    graph = MpmSimulation(cfg, max_particles=len(x))
+    for s in (eager, graph):
+        s.set_particles(x)
+        s.set_p2g_mode("deterministic")
+    for rep in range(4):
+        log(f"graph replay {rep}")
+        eager.step_n(dt, 5)
+        graph.step_n_graph(dt, 5)
+        a, b = eager.read_positions(), graph.read_positions()
+        assert np.array_equal(a, b)
+        _ = eager.read_field("F")  # eager work between replays
+
+
+def test_fault_latch_freezes_state():
+    cfg = MpmConfig.sand()
+    x = fill_box([-0.1, 0.3, -0.1], [0.1, 0.5, 0.1], 4, 2)
+    v = np.zeros_like(x)
+    v[3, 0] = np.nan
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x, v)
+    sim.step_n(1 / 240, 3)
+    st = sim.read_status()
+    assert st["numerical_fault"] and st["faulted"]
+    assert np.array_equal(sim.read_positions(), x)
+    sim.reset_status()
+    assert not sim.read_status()["faulted"]
+
+
+def test_closed_system_momentum_and_dropped_block():
+    # tests/mpm_gpu.rs:781-838 and tests/mpm_freefall_regression_gpu.rs:113-128
+    cfg = MpmConfig.sand().copy(gravity_y=0.0, grid_origin=(-1.6, -1.6, -1.6), grid_size=32,
+                                cell_size=0.1, boundary_friction=1.0)
+    x = fill_box([-0.4] * 3, [0.4] * 3, 8, 7)
+    _, r = run_gpu(cfg, x, None, 1 / 240, 60)
+    p = np.linalg.norm(r["v"].astype(np.float64).sum(0)) * cfg.particle_mass
+    assert p < 0.01 * cfg.particle_mass * len(x)
+    cfg = MpmConfig.snow()
+    x = fill_box([-0.075, 0.12, -0.075], [0.075, 0.27, 0.075], 6, 22)
+    for mode in ("fused", "global_atomic"):
+        _, r = run_gpu(cfg, x, None, 1 / 480, 300, mode=mode)
+        assert r["x"][:, 1].max() < 0.35 and np.abs(r["x"][:, 0]).max() < 0.25
+
+
+def test_batched_scenes_match_oracle_per_scene():
+    cfgs, xs, meta = workloads.batched_scenes(6, seed=11, n_lo=500, n_hi=4000, g_lo=32, g_hi=48)
+    log(f"batch of {len(cfgs)} scenes, {meta['particles']} particles")
+    sim = MpmSimulation(cfgs, max_particles=None)
+    sim.set_particles_batch(xs)
+    dt, steps = meta["dt"], 30
+    sim.step_n(dt, steps)
+    pos = sim.read_positions()
+    off = 0
+    for c, x in zip(cfgs, xs):
+        o64 = oracle_state(c, x)
+        o32 = oracle_state(c, x, dtype=np.float32)
+        orc.step(c, o64, dt, steps)
+        orc.step(c, o32, dt, steps)
+        e = rel_err(pos[off:off + len(x)], o64.x)
+        e32 = rel_err(o32.x, o64.x)
+        log(f"  scene n={len(x)} G={c.grid_size}: x err {e:.2e} (f32 oracle {e32:.2e})")
+        assert e <= max(8 * e32, 1e-6)
+        off += len(x)
+
+
+def test_cpu_diff_model_forward_matches_oracle():
+    cfg, d, meta = workloads.diff_scene(4096, seed=6, grid=32)
+    x, v = d["x"], d["v"]
+    log("cpudiff forward")
+    _, g = run_gpu(cfg, x, v, meta["dt"], 16, fields={"youngs": d["E"]})
+    st = oracle_state(cfg, x, v, E=d["E"].astype(np.float64))
+    orc.diff_forward(cfg, st, meta["dt"], 16, record=False)
+    ex, ev, eF = rel_err(g["x"], st.x), rel_err(g["v"], st.v), rel_err(g["F"], st.F)
+    log(f"  x {ex:.2e} v {ev:.2e} F {eF:.2e}")
+    assert ex < 1e-6 and ev < 1e-3 and eF < 1e-4
