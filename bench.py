@@ -135,6 +135,7 @@ def main():
                 "algorithmic_bytes_per_launch": BYTES_FWD * n,
                 "stage_ms_per_substep": {k: (v[0] / max(v[1], 1)) for k, v in prof.items()}}
     log(f"profile: {prof}")
+    log(f"status: {st}")
 
     out = {"metric": METRIC, "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps,

@@ -101,6 +101,10 @@ typedef struct {
     uint32_t completed_gathers; /* G2P passes run (g_kdiag metadata word 3)   */
     uint32_t active_tiles;      /* 4^3 tiles touched by the last P2G          */
     uint32_t sorts;             /* particle re-sorts performed                */
+    uint32_t escapes;           /* particle-substeps scattered by global atomics because the
+                                   particle left its sorted tile neighbourhood */
+    uint32_t fallback_chunks;   /* block passes whose P2G box exceeded the LDS budget */
+    uint32_t chunks;            /* blocks of the sorted layout's fused pass   */
 } mpm_status_info;
 
 /* ---- lifetime (MpmSimulation::new / new_with_device, mpm.rs:834-853) ---- */

@@ -61,6 +61,19 @@ struct mpm_sim {
     float4* gv = nullptr;
     uint32_t* tile_flag = nullptr;
     uint32_t* tile_scene = nullptr;
+    uint32_t* tile_lb = nullptr;
+    uint32_t* slist = nullptr;
+    uint32_t* src_slot = nullptr;
+    int4* chunk_box = nullptr;
+    float4* part = nullptr;
+    uint32_t max_chunks = 0;   // host chunk list (scene-aligned, unsorted layout)
+    Chunk* schunks = nullptr;  // device-built chunk list of the sorted layout
+    uint32_t max_schunks = 0;
+    uint32_t part_cap = 0;     // chunks with an LDS partial slot
+    uint32_t* chunk_of_slot = nullptr;
+    uint32_t* cflags = nullptr;
+    uint32_t* cexcl = nullptr;
+    uint32_t* scan_scratch = nullptr;
     uint32_t* list[2] = {nullptr, nullptr};
     uint32_t* status = nullptr;  // ST_WORDS u32
     uint32_t* slot_of_id = nullptr;
@@ -69,8 +82,6 @@ struct mpm_sim {
     uint32_t* sort_scratch = nullptr;
     Chunk* d_chunks = nullptr;
     uint32_t n_chunks = 0;
-    uint32_t* tile_begin = nullptr;
-    uint32_t* tile_end = nullptr;
     float* affine = nullptr;
     float* staging = nullptr;  // cap * 9 floats
     int list_cur = 0;
@@ -178,7 +189,9 @@ void release(mpm_sim* s) {
     }
     void* bufs[] = {s->d_scenes, s->d_mats, s->ps[0], s->ps[1], s->gmp, s->gv, s->tile_flag, s->tile_scene,
                     s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
-                    s->d_chunks, s->tile_begin, s->tile_end, s->affine, s->staging};
+                    s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
+                    s->src_slot, s->chunk_box, s->part, s->schunks, s->chunk_of_slot, s->cflags,
+                    s->cexcl, s->scan_scratch};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
@@ -194,16 +207,37 @@ mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
     a.gmp = s->gmp;
     a.gv = s->gv;
     a.tile_flag = s->tile_flag;
+    a.sorted_keys = s->keys;
+    a.chunk_box = s->chunk_box;
+    a.part = s->part;
     a.list_out = s->list[list_out];
     a.count_out = s->status + mpm::ST_COUNT0 + list_out;
     a.status = s->status;
     a.chunks = s->d_chunks;
     a.n_chunks = s->n_chunks;
+    a.n_chunks_dev = nullptr;
+    a.part_capacity = s->part_cap;
     a.dt = dt;
     a.use_lds = s->p2g_mode == MPM_P2G_FUSED ? 1 : 0;
     a.write_vc = s->write_vc;
     a.has_E = s->has_E;
     return a;
+}
+
+mpm::GridDev griddev(mpm_sim* s) {
+    mpm::GridDev g{};
+    g.gmp = s->gmp;
+    g.gv = s->gv;
+    g.tile_flag = s->tile_flag;
+    g.tile_scene = s->tile_scene;
+    g.tile_lb = s->tile_lb;
+    g.chunk_box = s->chunk_box;
+    g.part = s->part;
+    g.slist = s->slist;
+    g.scount = s->status + mpm::ST_SCOUNT;
+    g.chunk_of_slot = s->chunk_of_slot;
+    g.part_capacity = s->part_cap;
+    return g;
 }
 
 // ---- profiling helpers ----
@@ -258,6 +292,18 @@ int ensure_grid(mpm_sim* s) {
     HIP_TRY(s, dalloc(&s->tile_scene, s->total_tiles));
     HIP_TRY(s, dalloc(&s->list[0], s->total_tiles));
     HIP_TRY(s, dalloc(&s->list[1], s->total_tiles));
+    HIP_TRY(s, dalloc(&s->tile_lb, (size_t)s->total_tiles + 1));
+    HIP_TRY(s, dalloc(&s->slist, s->total_tiles));
+    s->part_cap = s->cap / 256 + s->cap / 64 + s->n_scenes + 64;
+    s->max_schunks = s->cap + s->cap / 256 + s->n_scenes + 64;
+    HIP_TRY(s, dalloc(&s->chunk_box, 2 * (size_t)s->part_cap));
+    HIP_TRY(s, dalloc(&s->part, (size_t)s->part_cap * mpm::LDS_CELLS));
+    HIP_TRY(s, dalloc(&s->schunks, s->max_schunks));
+    HIP_TRY(s, dalloc(&s->chunk_of_slot, s->cap));
+    HIP_TRY(s, dalloc(&s->cflags, s->cap));
+    HIP_TRY(s, dalloc(&s->cexcl, s->cap));
+    HIP_TRY(s, dalloc(&s->scan_scratch, mpm::scan_scratch_words(s->cap)));
+    HIP_TRY(s, hipMemsetAsync(s->chunk_box, 0, 2 * (size_t)s->part_cap * sizeof(int4), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gmp, 0, cells * sizeof(float4), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gv, 0, cells * sizeof(float4), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->tile_flag, 0, (size_t)s->total_tiles * 4, s->stream));
@@ -268,9 +314,10 @@ int ensure_grid(mpm_sim* s) {
         std::fill(ts.begin() + s->sh[i].tile_base, ts.begin() + s->sh[i].tile_base + W * W * W, i);
     }
     HIP_TRY(s, hipMemcpy(s->tile_scene, ts.data(), ts.size() * 4, hipMemcpyHostToDevice));
-    uint32_t z[2] = {0u, 0u};
+    uint32_t z[4] = {0u, 0u, 0u, 0u};
     HIP_TRY(s, hipMemcpy(s->status + mpm::ST_COUNT0, z, sizeof(z), hipMemcpyHostToDevice));
     s->grid_dirty = false;
+    s->sorted = false;
     return MPM_OK;
 }
 
@@ -280,8 +327,8 @@ int clear_grid(mpm_sim* s) {
         s->grid_dirty = false;
         return MPM_OK;
     }
-    HIP_TRY(s, mpm::launch_clear_tiles(s->gmp, s->tile_flag, s->list[s->list_cur],
-                                       s->status + mpm::ST_COUNT0 + s->list_cur, grid_blocks(s), s->stream));
+    HIP_TRY(s, mpm::launch_clear_tiles(griddev(s), s->list[s->list_cur], s->status + mpm::ST_COUNT0 + s->list_cur,
+                                       grid_blocks(s), s->stream));
     s->grid_dirty = false;
     return MPM_OK;
 }
@@ -311,9 +358,15 @@ int sort_particles(mpm_sim* s, bool reorder) {
     HIP_TRY(s, mpm::radix_sort_pairs(s->keys, s->vals, s->n, bits, s->sort_scratch, s->stream));
     if (reorder) {
         const int other = 1 - s->ps_cur;
-        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->vals, s->slot_of_id, s->n,
-                                       s->stream));
+        HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
+        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->src_slot, s->n, s->stream));
         HIP_TRY(s, mpm::launch_index_slots(s->ps[other], s->cap, s->slot_of_id, s->n, s->stream));
+        HIP_TRY(s, mpm::launch_static_rebuild(s->d_scenes, s->tile_scene, s->tile_flag, s->slist,
+                                              s->status + mpm::ST_SCOUNT, s->keys, s->n, s->tile_lb,
+                                              s->total_tiles, grid_blocks(s), s->stream));
+        HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
+                                            s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
+                                            s->chunk_of_slot, s->stream));
         s->ps_cur = other;
         s->sorted = true;
         s->since_sort = 0;
@@ -326,7 +379,15 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out) {
     if (s->dry) return MPM_OK;
     Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
     mpm::StepArgs a = args(s, dt, list_out);
-    HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, particle_blocks(s), s->stream));
+    int blocks = particle_blocks(s);
+    if (s->p2g_mode == MPM_P2G_FUSED && s->sorted) {
+        // sorted layout: the device-built chunk list, walked by a persistent grid
+        a.chunks = s->schunks;
+        a.n_chunks_dev = s->status + mpm::ST_NCHUNKS;
+        a.n_chunks = std::min<uint32_t>(s->max_schunks, s->n / 256 + s->n / 64 + s->n_scenes + 64);
+        blocks = s->pblocks ? int(s->pblocks) : int(std::min<uint32_t>(a.n_chunks, 4096u));
+    }
+    HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, blocks, s->stream));
     return MPM_OK;
 }
 
@@ -334,31 +395,22 @@ int grid_update(mpm_sim* s, float dt, bool zero_after) {
     if (s->dry) return MPM_OK;
     Stage st(s, MPM_STAGE_GRID);
     const int nxt = 1 - s->list_cur;
-    HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, s->tile_scene, s->gmp, s->gv, s->tile_flag,
-                                       s->list[s->list_cur], s->status + mpm::ST_COUNT0 + s->list_cur,
-                                       zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, s->status, dt,
-                                       zero_after ? 1 : 0, 0, grid_blocks(s), s->stream));
+    HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, griddev(s), s->list[s->list_cur],
+                                       s->status + mpm::ST_COUNT0 + s->list_cur,
+                                       zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, dt,
+                                       zero_after ? 1 : 0, s->p2g_mode == MPM_P2G_FUSED ? 1 : 0,
+                                       grid_blocks(s), s->stream));
     return MPM_OK;
 }
 
 int deterministic_p2g(mpm_sim* s, float dt) {
     if (s->dry) return MPM_OK;
     Stage st(s, MPM_STAGE_P2G);
-    if (!s->tile_begin) {
-        HIP_TRY(s, dalloc(&s->tile_begin, s->total_tiles));
-        HIP_TRY(s, dalloc(&s->tile_end, s->total_tiles));
-        HIP_TRY(s, dalloc(&s->affine, (size_t)12 * s->cap));
-    }
-    HIP_TRY(s, mpm::launch_fill_u32(s->tile_begin, 0u, s->total_tiles, s->stream));
-    HIP_TRY(s, mpm::launch_fill_u32(s->tile_end, 0u, s->total_tiles, s->stream));
-    HIP_TRY(s, mpm::launch_tile_ranges(s->keys, s->n, s->tile_begin, s->tile_end, s->stream));
-    uint32_t* cnt = s->status + mpm::ST_COUNT0 + s->list_cur;
-    HIP_TRY(s, mpm::launch_mark_dilated(s->d_scenes, s->tile_scene, s->keys, s->n, s->tile_flag,
-                                        s->list[s->list_cur], cnt, s->stream));
+    if (!s->affine) HIP_TRY(s, dalloc(&s->affine, (size_t)12 * s->cap));
     mpm::StepArgs a = args(s, dt, s->list_cur);
     HIP_TRY(s, mpm::launch_gather_prepare(a, s->affine, particle_blocks(s), s->stream));
-    HIP_TRY(s, mpm::launch_gather_p2g(a, s->tile_scene, s->tile_begin, s->tile_end, s->affine,
-                                      s->list[s->list_cur], cnt, 4096, s->stream));
+    HIP_TRY(s, mpm::launch_gather_p2g(a, s->tile_scene, s->tile_lb, s->affine, s->slist,
+                                      s->status + mpm::ST_SCOUNT, 4096, s->stream));
     return MPM_OK;
 }
 
@@ -424,6 +476,7 @@ int field_layout(int field, int* comp0, int* ncomp) {
 int build_chunks(mpm_sim* s) {
     std::vector<Chunk> ch;
     for (uint32_t sc = 0; sc < s->n_scenes; ++sc) {
+        s->sh[sc].cbase = uint32_t(ch.size());
         for (uint32_t off = 0; off < s->counts[sc]; off += CHUNK) {
             Chunk c{};
             c.scene = sc;
@@ -496,8 +549,10 @@ int create(const mpm_scene_config* cfgs, uint32_t n_scenes, mpm_sim** out) {
     HIP_TRY(s, dalloc(&s->slot_of_id, s->cap));
     HIP_TRY(s, dalloc(&s->keys, s->cap));
     HIP_TRY(s, dalloc(&s->vals, s->cap));
+    HIP_TRY(s, dalloc(&s->src_slot, s->cap));
     HIP_TRY(s, dalloc(&s->sort_scratch, mpm::radix_scratch_words(s->cap)));
-    HIP_TRY(s, dalloc(&s->d_chunks, (size_t)(s->cap / CHUNK + n_scenes + 1)));
+    s->max_chunks = s->cap / CHUNK + n_scenes + 1;
+    HIP_TRY(s, dalloc(&s->d_chunks, (size_t)s->max_chunks));
     HIP_TRY(s, dalloc(&s->staging, (size_t)9 * s->cap));
     HIP_TRY(s, hipMemsetAsync(s->status, 0, mpm::ST_WORDS * 4, s->stream));
     HIP_TRY(s, hipMemsetAsync(s->ps[0], 0, (size_t)mpm::NCOMP * s->cap * 4, s->stream));
@@ -739,11 +794,8 @@ int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
             int rc0 = ensure_grid(sim);
             if (rc0 != MPM_OK) return rc0;
         }
-        if (sim->p2g_mode == MPM_P2G_DETERMINISTIC && !sim->tile_begin) {
-            HIP_TRY(sim, dalloc(&sim->tile_begin, sim->total_tiles));
-            HIP_TRY(sim, dalloc(&sim->tile_end, sim->total_tiles));
+        if (sim->p2g_mode == MPM_P2G_DETERMINISTIC && !sim->affine)
             HIP_TRY(sim, dalloc(&sim->affine, (size_t)12 * sim->cap));
-        }
         hipGraph_t g;
         HIP_TRY(sim, hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal));
         sim->capturing = true;
@@ -803,6 +855,9 @@ int mpm_read_status(mpm_sim* sim, mpm_status_info* out) {
     out->completed_gathers = w[mpm::ST_GATHERS];
     out->active_tiles = w[mpm::ST_COUNT0 + sim->list_cur];
     out->sorts = sim->sorts;
+    out->escapes = w[mpm::ST_ESCAPES];
+    out->fallback_chunks = w[mpm::ST_FALLBACK];
+    out->chunks = w[mpm::ST_NCHUNKS];
     return MPM_OK;
 }
 
@@ -811,6 +866,8 @@ int mpm_reset_status(mpm_sim* sim) {
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
     uint32_t z[2] = {0u, 0u};
     HIP_TRY(sim, hipMemcpy(sim->status, z, sizeof(z), hipMemcpyHostToDevice));
+    HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_ESCAPES, z, 4, hipMemcpyHostToDevice));
+    HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_FALLBACK, z, 4, hipMemcpyHostToDevice));
     return MPM_OK;
 }
 
@@ -822,6 +879,7 @@ int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_
     if (sim->n > 0) {
         int rc = sort_particles(sim, false);
         if (rc != MPM_OK) return rc;
+        sim->sorted = false;  // keys now describe unsorted positions: re-sort before stepping
         HIP_TRY(sim, hipMemcpyAsync(keys, sim->keys + sim->pbase[scene], (size_t)cnt * 4, hipMemcpyDeviceToHost,
                                     sim->stream));
         HIP_TRY(sim, hipMemcpyAsync(ids, sim->vals + sim->pbase[scene], (size_t)cnt * 4, hipMemcpyDeviceToHost,

@@ -33,7 +33,10 @@ constexpr uint32_t FAULT_OVERFLOW = 1u;
 constexpr uint32_t FAULT_NUMERICAL = 2u;
 
 // Status words in device memory.
-enum StatusWord : int { ST_FAULT = 0, ST_GATHERS = 1, ST_COUNT0 = 2, ST_COUNT1 = 3, ST_WORDS = 8 };
+// ST_COUNT0/1: the two dynamic active-tile lists; ST_SCOUNT: the static list.
+enum StatusWord : int {
+    ST_FAULT = 0, ST_GATHERS = 1, ST_COUNT0 = 2, ST_COUNT1 = 3, ST_SCOUNT = 4, ST_ESCAPES = 5, ST_NCHUNKS = 6, ST_FALLBACK = 7, ST_WORDS = 8
+};
 
 struct SceneDev {
     float origin[3];
@@ -56,6 +59,7 @@ struct SceneDev {
     uint32_t n_mat;
     uint32_t pbase;
     uint32_t n;
+    uint32_t cbase;  // first chunk of the scene
 };
 
 // A contiguous run of particle slots of one scene, processed by one block.
@@ -66,9 +70,40 @@ struct Chunk {
     uint32_t pad;
 };
 
+constexpr uint32_t CHUNK_PARTICLES = 256;  // one particle per thread of a block
+
 constexpr int MAT_STRIDE = 8;  // rheology, dp_alpha, theta_c, theta_s, hardening, vdamp, adamp, bulk
 
 enum Mode : int { MODE_P2G = 0, MODE_G2P = 1, MODE_G2P2G = 2 };
+
+// Tile flag bits.
+constexpr uint32_t FLAG_DYN = 1u;     // on the dynamic active list being built
+constexpr uint32_t FLAG_GMP = 2u;     // gmp holds atomically accumulated data
+constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last sort
+
+// P2G partials.  Each block of the fused particle pass reduces its scatter in
+// LDS over the bounding box of its particles' stencils and stores that box as
+// its chunk's partial (plain stores).  Particles are kept sorted by 4^3 tile,
+// so the chunks that can touch tile T are the chunks holding particles sorted
+// into T's 3x3x3 tile neighbourhood; the grid update finds them from the
+// tile lower-bound table of the last sort and sums their partials per cell.
+// A particle whose stencil has left that neighbourhood since the sort is
+// scattered with global atomics instead (and counted in ST_ESCAPES).
+constexpr int LDS_CELLS = 2048;
+
+struct GridDev {
+    float4* gmp;          // atomic accumulation / final P2G kept for readback
+    float4* gv;           // grid velocity (vx, vy, vz, m)
+    uint32_t* tile_flag;  // FLAG_* bits
+    const uint32_t* tile_scene;
+    const uint32_t* tile_lb;  // first sorted slot with key >= tile (total_tiles + 1)
+    int4* chunk_box;      // per chunk: (x0, y0, z0, -), (bx, by, bz, vol); bx = 0: empty
+    float4* part;         // per chunk: LDS_CELLS cells (m, px, py, pz)
+    const uint32_t* slist;  // static active list
+    const uint32_t* scount;
+    const uint32_t* chunk_of_slot;  // sorted slot -> chunk of the fused pass
+    uint32_t part_capacity;         // chunks with a partial slot
+};
 
 struct StepArgs {
     const SceneDev* scenes;
@@ -78,11 +113,16 @@ struct StepArgs {
     float4* gmp;
     float4* gv;
     uint32_t* tile_flag;
+    const uint32_t* sorted_keys;  // tile key of each slot at the last sort
+    int4* chunk_box;
+    float4* part;
     uint32_t* list_out;   // active-tile list being built by the scatter
     uint32_t* count_out;  // its counter
     uint32_t* status;
     const Chunk* chunks;
-    uint32_t n_chunks;
+    uint32_t n_chunks;              // launch bound (and count when n_chunks_dev is null)
+    const uint32_t* n_chunks_dev;   // device-built chunk count of the sorted layout
+    uint32_t part_capacity;         // chunks >= this scatter with global atomics
     float dt;
     int use_lds;   // 0: every node goes straight to global atomics
     int write_vc;  // 1: store v and C every substep
@@ -92,17 +132,20 @@ struct StepArgs {
 // ---- launchers (mpm_kernels.hip) ----
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
                                 hipStream_t s);
-hipError_t launch_grid_update(const SceneDev* scenes, const uint32_t* tile_scene, float4* gmp,
-                              float4* gv, uint32_t* tile_flag, const uint32_t* list,
-                              const uint32_t* count, uint32_t* next_count, uint32_t* status,
-                              float dt, int zero_after, int count_gather, int blocks,
+hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
+                              const uint32_t* dcount, uint32_t* next_count, float dt,
+                              int zero_after, int use_partials, int blocks, hipStream_t s);
+hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s);
-hipError_t launch_clear_tiles(float4* gmp, uint32_t* tile_flag, const uint32_t* list,
-                              uint32_t* count, int blocks, hipStream_t s);
+// Static active list of a sort: the 3x3x3 dilation of occupied tiles.
+hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_scene,
+                                 uint32_t* tile_flag, uint32_t* slist, uint32_t* scount,
+                                 const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_lb,
+                                 uint32_t total_tiles, int blocks, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t s);
-hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* sorted_ids,
-                          const uint32_t* slot_of_id, uint32_t n, hipStream_t s);
+hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
+                          uint32_t n, hipStream_t s);
 hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,
                               hipStream_t s);
 hipError_t launch_unpermute(const float* ps, uint32_t cap, int comp0, int ncomp,
@@ -113,16 +156,21 @@ hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
 hipError_t launch_grid_to_dense(const float4* gmp, const SceneDev* scenes, uint32_t scene,
                                 uint32_t G, float* mass, float* mom, hipStream_t s);
 // Deterministic gather-order P2G.
-hipError_t launch_tile_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_begin,
-                              uint32_t* tile_end, hipStream_t s);
-hipError_t launch_mark_dilated(const SceneDev* scenes, const uint32_t* tile_scene,
-                               const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_flag,
-                               uint32_t* list, uint32_t* count, hipStream_t s);
 hipError_t launch_gather_prepare(const StepArgs& a, float* affine, int blocks, hipStream_t s);
-hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene,
-                             const uint32_t* tile_begin, const uint32_t* tile_end,
-                             const float* affine, const uint32_t* list, const uint32_t* count,
+hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene, const uint32_t* tile_lb,
+                             const float* affine, const uint32_t* slist, const uint32_t* scount,
                              int blocks, hipStream_t s);
+hipError_t launch_reorder_index(const uint32_t* sorted_ids, const uint32_t* slot_of_id,
+                                uint32_t* src_slot, uint32_t n, hipStream_t s);
+
+// ---- scans and the sorted chunk list (mpm_scan.hip) ----
+size_t scan_scratch_words(size_t n);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
+                              uint32_t** total, hipStream_t s);
+hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
+                               const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
+                               uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
+                               uint32_t* chunk_of_slot, hipStream_t s);
 
 // ---- radix sort (mpm_sort.hip) ----
 size_t radix_scratch_words(uint32_t n);

@@ -11,6 +11,8 @@
 // particles' stencils (particles are kept sorted by 4^3 tile, so that box is
 // small), then flushes each touched cell with one global f32 atomic per
 // channel and appends newly touched tiles to the next active list.
+#include <algorithm>
+
 #include "mpm_device.hpp"
 #include "mpm_internal.hpp"
 
@@ -18,7 +20,6 @@ namespace mpm {
 
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
-constexpr int LDS_CELLS = 2048;  // 4 planes x 2048 x 4 B = 32 KiB per block
 
 __device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int iy, int iz) {
     uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
@@ -38,12 +39,17 @@ __device__ __forceinline__ void global_add(float* p, float v) {
 }
 
 __device__ __forceinline__ void mark_tile(uint32_t* flag, uint32_t* list, uint32_t* count, uint32_t t) {
-    if (__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        if (atomicExch(&flag[t], 1u) == 0u) {
+    if ((__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (FLAG_DYN | FLAG_STATIC)) == 0u) {
+        if ((atomicOr(&flag[t], FLAG_DYN) & (FLAG_DYN | FLAG_STATIC)) == 0u) {
             uint32_t slot = atomicAdd(count, 1u);
             list[slot] = t;
         }
     }
+}
+
+__device__ __forceinline__ void mark_gmp(uint32_t* flag, uint32_t t) {
+    if ((__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_GMP) == 0u)
+        atomicOr(&flag[t], FLAG_GMP);
 }
 
 __device__ __forceinline__ void latch(uint32_t* status, uint32_t bits) {
@@ -68,7 +74,8 @@ __device__ __forceinline__ const float* mat_row(const StepArgs& a, const SceneDe
 }
 
 struct Scatter {
-    bool on;
+    bool on;   // the particle scatters this pass
+    bool esc;  // ... with global atomics (stencil left its sort neighbourhood)
     int b[3];
     float fx[3];
     float m;
@@ -76,24 +83,61 @@ struct Scatter {
     float A[9];  // m C + 4/dx^2 stress
 };
 
-// P2G of one block's particles.  Every thread of the block must call this.
+// LDS workspace of the block scatter.
+struct ScatterLds {
+    float acc[4][LDS_CELLS];  // (m, px, py, pz) planes of the block's box: 32 KiB
+    int red[WAVES * 6];
+    int box[16];
+};
+
+// One particle's 27-node P2G straight to global f32 atomics.
+__device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev& sc, const Scatter& p) {
+    float w[3][3];
+    quad_weights(p.fx[0], w[0]);
+    quad_weights(p.fx[1], w[1]);
+    quad_weights(p.fx[2], w[2]);
+    const float dx = sc.dx;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) {
+                int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
+                if (!in_grid(sc, ix, iy, iz)) continue;
+                float wt = w[0][i] * w[1][j] * w[2][k];
+                float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
+                      d2 = (float(k) - p.fx[2]) * dx;
+                uint32_t c = cell_index(sc, ix, iy, iz);
+                float* g = reinterpret_cast<float*>(&a.gmp[c]);
+                global_add(g + 0, wt * p.m);
+                global_add(g + 1, wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
+                global_add(g + 2, wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
+                global_add(g + 3, wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
+                mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
+                mark_gmp(a.tile_flag, c >> 6);
+            }
+}
+
+// P2G of one block's particles (chunk `chunk`).  Every thread of the block
+// must call this.  Contributions are reduced with LDS f32 atomics over the
+// bounding box of the block's stencils; the box is stored as the chunk's
+// partial with plain stores (no global atomics, no registration: the grid
+// update finds the chunk through the sort).  Escaped particles, and blocks
+// whose box exceeds LDS_CELLS, scatter with global atomics.
 __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
-                                              float* s_acc, int* s_red, int* s_box) {
+                                              uint32_t chunk, ScatterLds& L) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float dx = sc.dx;
+    const bool lds_on = p.on && !p.esc;
     int lo[3], hi[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        lo[k] = p.on ? p.b[k] : 0x3fffffff;
-        hi[k] = p.on ? p.b[k] + 2 : -0x3fffffff;
-        lo[k] = wave_min(lo[k]);
-        hi[k] = wave_max(hi[k]);
+        lo[k] = wave_min(lds_on ? p.b[k] : 0x3fffffff);
+        hi[k] = wave_max(lds_on ? p.b[k] + 2 : -0x3fffffff);
     }
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            s_red[wave * 6 + k] = lo[k];
-            s_red[wave * 6 + 3 + k] = hi[k];
+            L.red[wave * 6 + k] = lo[k];
+            L.red[wave * 6 + 3 + k] = hi[k];
         }
     }
     __syncthreads();
@@ -101,47 +145,50 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         int l[3], h[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            l[k] = s_red[k];
-            h[k] = s_red[3 + k];
+            l[k] = L.red[k];
+            h[k] = L.red[3 + k];
             for (int w = 1; w < WAVES; ++w) {
-                l[k] = min(l[k], s_red[w * 6 + k]);
-                h[k] = max(h[k], s_red[w * 6 + 3 + k]);
+                l[k] = min(l[k], L.red[w * 6 + k]);
+                h[k] = max(h[k], L.red[w * 6 + 3 + k]);
             }
             l[k] = max(l[k], 0);
             h[k] = min(h[k], int(sc.G) - 1);
         }
-        int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
-        int use = 0;
-        if (a.use_lds && bx > 0 && by > 0 && bz > 0 && bx * by * bz <= LDS_CELLS) use = 1;
-        if (bx <= 0 || by <= 0 || bz <= 0) use = 2;  // nothing to scatter
-        s_box[0] = l[0];
-        s_box[1] = l[1];
-        s_box[2] = l[2];
-        s_box[3] = bx;
-        s_box[4] = by;
-        s_box[5] = bz;
-        s_box[6] = use;
+        const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
+        int use = 2;  // nothing reduced in LDS
+        if (bx > 0 && by > 0 && bz > 0)
+            use = (a.use_lds && bx * by * bz <= LDS_CELLS && chunk < a.part_capacity) ? 1 : 0;
+        L.box[0] = l[0];
+        L.box[1] = l[1];
+        L.box[2] = l[2];
+        L.box[3] = bx;
+        L.box[4] = by;
+        L.box[5] = bz;
+        L.box[6] = use;
+        if (use == 0 && a.use_lds) atomicAdd(&a.status[ST_FALLBACK], 1u);
+        if (a.use_lds && chunk < a.part_capacity) {
+            a.chunk_box[2 * (size_t)chunk] = make_int4(l[0], l[1], l[2], 0);
+            a.chunk_box[2 * (size_t)chunk + 1] =
+                use == 1 ? make_int4(bx, by, bz, bx * by * bz) : make_int4(0, 0, 0, 0);
+        }
     }
     __syncthreads();
-    const int use = s_box[6];
-    if (use == 2) return;
-    float w[3][3];
-    if (p.on) {
-        quad_weights(p.fx[0], w[0]);
-        quad_weights(p.fx[1], w[1]);
-        quad_weights(p.fx[2], w[2]);
-    }
+    const int use = L.box[6];
     if (use == 1) {
-        const int x0 = s_box[0], y0 = s_box[1], z0 = s_box[2], bx = s_box[3], by = s_box[4];
-        const int vol = bx * by * s_box[5];
+        const int x0 = L.box[0], y0 = L.box[1], z0 = L.box[2], bx = L.box[3], by = L.box[4];
+        const int vol = bx * by * L.box[5];
         for (int i = tid; i < vol; i += BLOCK) {
-            s_acc[i] = 0.0f;
-            s_acc[LDS_CELLS + i] = 0.0f;
-            s_acc[2 * LDS_CELLS + i] = 0.0f;
-            s_acc[3 * LDS_CELLS + i] = 0.0f;
+            L.acc[0][i] = 0.0f;
+            L.acc[1][i] = 0.0f;
+            L.acc[2][i] = 0.0f;
+            L.acc[3][i] = 0.0f;
         }
         __syncthreads();
-        if (p.on) {
+        if (lds_on) {
+            float w[3][3];
+            quad_weights(p.fx[0], w[0]);
+            quad_weights(p.fx[1], w[1]);
+            quad_weights(p.fx[2], w[2]);
             for (int k = 0; k < 3; ++k)
                 for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -152,45 +199,20 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                         float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
                               d2 = (float(k) - p.fx[2]) * dx;
                         int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-                        lds_add(&s_acc[li], wt * p.m);
-                        lds_add(&s_acc[LDS_CELLS + li], wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
-                        lds_add(&s_acc[2 * LDS_CELLS + li], wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
-                        lds_add(&s_acc[3 * LDS_CELLS + li], wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
+                        lds_add(&L.acc[0][li], wt * p.m);
+                        lds_add(&L.acc[1][li], wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
+                        lds_add(&L.acc[2][li], wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
+                        lds_add(&L.acc[3][li], wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
                     }
         }
         __syncthreads();
-        for (int i = tid; i < vol; i += BLOCK) {
-            float m = s_acc[i], px = s_acc[LDS_CELLS + i], py = s_acc[2 * LDS_CELLS + i],
-                  pz = s_acc[3 * LDS_CELLS + i];
-            if (m == 0.0f && px == 0.0f && py == 0.0f && pz == 0.0f) continue;
-            int lx = i % bx, ly = (i / bx) % by, lz = i / (bx * by);
-            int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
-            uint32_t c = cell_index(sc, ix, iy, iz);
-            float* g = reinterpret_cast<float*>(&a.gmp[c]);
-            global_add(g + 0, m);
-            global_add(g + 1, px);
-            global_add(g + 2, py);
-            global_add(g + 3, pz);
-            mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
-        }
+        float4* part = a.part + (size_t)chunk * LDS_CELLS;
+        for (int i = tid; i < vol; i += BLOCK) part[i] = make_float4(L.acc[0][i], L.acc[1][i], L.acc[2][i], L.acc[3][i]);
         __syncthreads();
-    } else if (p.on) {
-        for (int k = 0; k < 3; ++k)
-            for (int j = 0; j < 3; ++j)
-                for (int i = 0; i < 3; ++i) {
-                    int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
-                    if (!in_grid(sc, ix, iy, iz)) continue;
-                    float wt = w[0][i] * w[1][j] * w[2][k];
-                    float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
-                          d2 = (float(k) - p.fx[2]) * dx;
-                    uint32_t c = cell_index(sc, ix, iy, iz);
-                    float* g = reinterpret_cast<float*>(&a.gmp[c]);
-                    global_add(g + 0, wt * p.m);
-                    global_add(g + 1, wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
-                    global_add(g + 2, wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
-                    global_add(g + 3, wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
-                    mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
-                }
+    }
+    if (p.on && (p.esc || use == 0)) {
+        if (p.esc) atomicAdd(&a.status[ST_ESCAPES], 1u);
+        scatter_global(a, sc, p);
     }
 }
 
@@ -232,23 +254,23 @@ __device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float*
 
 template <int MODE, int MODEL>
 __global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
-    __shared__ float s_acc[4 * LDS_CELLS];
-    __shared__ int s_red[WAVES * 6];
-    __shared__ int s_box[8];
-    if (threadIdx.x == 0) s_box[7] = a.status[ST_FAULT] != 0u ? 1 : 0;
+    __shared__ ScatterLds L;
+    if (threadIdx.x == 0) L.box[15] = a.status[ST_FAULT] != 0u ? 1 : 0;
     __syncthreads();
-    if (s_box[7]) return;  // block-uniform: a latched fault stops the pass
+    if (L.box[15]) return;  // block-uniform: a latched fault stops the pass
     if (MODE != MODE_P2G && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.status[ST_GATHERS], 1u);
     const uint32_t cap = a.cap;
     float* __restrict__ ps = a.ps;
     const float dt = a.dt;
-    for (uint32_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
+    const uint32_t nch = a.n_chunks_dev ? *a.n_chunks_dev : a.n_chunks;
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
         const bool active = threadIdx.x < ch.count;
         Scatter sp;
         sp.on = false;
+        sp.esc = false;
         if (active) {
             float x[3], F[9];
 #pragma unroll
@@ -509,9 +531,18 @@ __global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
                 }
                 sp.m = m;
                 sp.on = true;
+                sp.esc = !a.use_lds;
+                if (a.use_lds) {
+                    // stencil must stay inside the 3x3x3 tiles around the sorted tile
+                    const uint32_t tl = a.sorted_keys[slot] - sc.tile_base;
+                    const int tk[3] = {int(tl % sc.W), int((tl / sc.W) % sc.W), int(tl / (sc.W * sc.W))};
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        if (sp.b[k] < 4 * tk[k] - 4 || sp.b[k] > 4 * tk[k] + 5) sp.esc = true;
+                }
             }
         }
-        if (MODE != MODE_G2P) block_scatter(a, sc, sp, s_acc, s_red, s_box);
+        if (MODE != MODE_G2P) block_scatter(a, sc, sp, ci, L);
     }
 }
 
@@ -520,6 +551,7 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
     (void)ppt;
     if (a.n_chunks == 0) return hipSuccess;
     if (blocks <= 0 || uint32_t(blocks) > a.n_chunks) blocks = int(a.n_chunks);
+    blocks = std::max(blocks, 1);
     dim3 g(blocks), b(BLOCK);
 #define MPM_LAUNCH(M, D) hipLaunchKernelGGL((k_particles<M, D>), g, b, 0, s, a)
     if (model == MODEL_SHADER) {
@@ -537,97 +569,198 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
 
 // ---------------------------------------------------------------------------
 // Grid update over the active tiles (mpm.wgsl:1010-1058; CPUDIFF model:
-// mpm_cpu_diff.rs:513-542).  One wave per tile, one lane per cell.
-__global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes,
-                                                     const uint32_t* __restrict__ tile_scene,
-                                                     float4* __restrict__ gmp, float4* __restrict__ gv,
-                                                     uint32_t* __restrict__ tile_flag,
-                                                     const uint32_t* __restrict__ list,
-                                                     const uint32_t* __restrict__ count,
+// mpm_cpu_diff.rs:513-542), fused with the reduction of the P2G partials:
+// one wave per tile, one lane per cell.  The tiles are the static list of the
+// last sort followed by the dynamic list of tiles reached by global atomics.
+// The chunks that may hold partials for tile T are those of the particles
+// sorted into T's 3x3x3 neighbourhood: nine x-rows of consecutive tile keys,
+// i.e. nine ascending slot ranges read from the tile lower-bound table.
+__device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix, int iy, int iz, float dt,
+                                            float4& out) {
+    const float m = q.x;
+    out = make_float4(0.f, 0.f, 0.f, m);
+    const int n_ = int(sc.G);
+    if (sc.model == MODEL_SHADER) {
+        if (m > 1.0e-9f && m == m) {
+            float vx = q.y / m, vy = q.z / m, vz = q.w / m;
+            vy = vy + dt * sc.gravity_y;
+            const float bf = sc.boundary_friction;
+            const int pad = 2;
+            if (ix < pad) { vx = fmaxf(vx, 0.f); vy *= bf; vz *= bf; }
+            if (ix >= n_ - pad) { vx = fminf(vx, 0.f); vy *= bf; vz *= bf; }
+            if (iy < pad) { vy = fmaxf(vy, 0.f); vx *= bf; vz *= bf; }
+            if (iy >= n_ - pad) { vy = fminf(vy, 0.f); vx *= bf; vz *= bf; }
+            if (iz < pad) { vz = fmaxf(vz, 0.f); vx *= bf; vy *= bf; }
+            if (iz >= n_ - pad) { vz = fminf(vz, 0.f); vx *= bf; vy *= bf; }
+            out = make_float4(vx, vy, vz, m);
+        }
+    } else {
+        if (m > 0.0f) {
+            float vx = q.y / m, vy = q.z / m, vz = q.w / m;
+            vy += sc.gravity_y * dt;
+            const float node_y = sc.origin[1] + float(iy) * sc.dx;
+            if (node_y < sc.floor_y) {
+                if (vy < 0.f) vy = 0.f;
+                const float damp = 1.0f - sc.floor_friction;
+                vx *= damp;
+                vz *= damp;
+            }
+            out = make_float4(vx, vy, vz, m);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
+                                                     const uint32_t* __restrict__ dlist,
+                                                     const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
-                                                     int zero_after) {
-    const uint32_t n = *count;
+                                                     int zero_after, int use_partials) {
+    const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
-        const uint32_t t = list[i];
-        const SceneDev sc = scenes[tile_scene[t]];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+        const SceneDev sc = scenes[g.tile_scene[t]];
         const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
         const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
         const int ix = tx * 4 + (lane & 3), iy = ty * 4 + ((lane >> 2) & 3), iz = tz * 4 + (lane >> 4);
         const uint32_t c = (t << 6) | uint32_t(lane);
-        const float4 q = gmp[c];
-        const float m = q.x;
-        float4 out = make_float4(0.f, 0.f, 0.f, m);
-        const int n_ = int(sc.G);
-        if (sc.model == MODEL_SHADER) {
-            if (m > 1.0e-9f && m == m) {
-                float vx = q.y / m, vy = q.z / m, vz = q.w / m;
-                vy = vy + dt * sc.gravity_y;
-                const float bf = sc.boundary_friction;
-                const int pad = 2;
-                if (ix < pad) { vx = fmaxf(vx, 0.f); vy *= bf; vz *= bf; }
-                if (ix >= n_ - pad) { vx = fminf(vx, 0.f); vy *= bf; vz *= bf; }
-                if (iy < pad) { vy = fmaxf(vy, 0.f); vx *= bf; vz *= bf; }
-                if (iy >= n_ - pad) { vy = fminf(vy, 0.f); vx *= bf; vz *= bf; }
-                if (iz < pad) { vz = fmaxf(vz, 0.f); vx *= bf; vy *= bf; }
-                if (iz >= n_ - pad) { vz = fminf(vz, 0.f); vx *= bf; vy *= bf; }
-                out = make_float4(vx, vy, vz, m);
-            }
-        } else {
-            if (m > 0.0f) {
-                float vx = q.y / m, vy = q.z / m, vz = q.w / m;
-                vy += sc.gravity_y * dt;
-                float node_y = sc.origin[1] + float(iy) * sc.dx;
-                if (node_y < sc.floor_y) {
-                    if (vy < 0.f) vy = 0.f;
-                    float damp = 1.0f - sc.floor_friction;
-                    vx *= damp;
-                    vz *= damp;
+        const uint32_t flag = g.tile_flag[t];
+        float4 q = (flag & FLAG_GMP) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (use_partials) {
+            int last = -1;
+            const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int nz = tz + dz;
+                if (nz < 0 || nz >= W) continue;
+                for (int dy = -1; dy <= 1; ++dy) {
+                    const int ny = ty + dy;
+                    if (ny < 0 || ny >= W) continue;
+                    const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));
+                    const uint32_t s0 = g.tile_lb[row + xa], s1 = g.tile_lb[row + xb + 1];
+                    if (s0 >= s1) continue;
+                    const int c0 = max(int(g.chunk_of_slot[s0]), last + 1);
+                    const int c1 = min(int(g.chunk_of_slot[s1 - 1]), int(g.part_capacity) - 1);
+                    for (int chn = c0; chn <= c1; ++chn) {
+                        const int4 o = g.chunk_box[2 * (size_t)chn];
+                        const int4 d = g.chunk_box[2 * (size_t)chn + 1];
+                        const int lx = ix - o.x, ly = iy - o.y, lz = iz - o.z;
+                        if (lx >= 0 && ly >= 0 && lz >= 0 && lx < d.x && ly < d.y && lz < d.z) {
+                            const float4 r = g.part[(size_t)chn * LDS_CELLS + lx + d.x * (ly + d.y * lz)];
+                            q.x += r.x;
+                            q.y += r.y;
+                            q.z += r.z;
+                            q.w += r.w;
+                        }
+                    }
+                    last = max(last, c1);
                 }
-                out = make_float4(vx, vy, vz, m);
             }
         }
-        gv[c] = out;
+        float4 out;
+        cell_update(sc, q, ix, iy, iz, dt, out);
+        g.gv[c] = out;
         if (zero_after) {
-            gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane == 0) tile_flag[t] = 0u;
+            if (flag & FLAG_GMP) g.gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (lane == 0) g.tile_flag[t] = flag & FLAG_STATIC;
+        } else {
+            g.gmp[c] = q;  // the last P2G stays readable (read_grid_mass_momentum)
+            if (lane == 0) g.tile_flag[t] = flag | FLAG_GMP;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_count) *next_count = 0u;
 }
 
-hipError_t launch_grid_update(const SceneDev* scenes, const uint32_t* tile_scene, float4* gmp,
-                              float4* gv, uint32_t* tile_flag, const uint32_t* list,
-                              const uint32_t* count, uint32_t* next_count, uint32_t* status,
-                              float dt, int zero_after, int count_gather, int blocks,
-                              hipStream_t s) {
-    (void)status;
-    (void)count_gather;
-    hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, gmp, gv,
-                       tile_flag, list, count, next_count, dt, zero_after);
+hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
+                              const uint32_t* dcount, uint32_t* next_count, float dt,
+                              int zero_after, int use_partials, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count, dt,
+                       zero_after, use_partials);
     return hipGetLastError();
 }
 
-// Zero the accumulation grid and the flags of the listed tiles; reset count.
-__global__ __launch_bounds__(256) void k_clear_tiles(float4* __restrict__ gmp, uint32_t* __restrict__ tile_flag,
-                                                     const uint32_t* __restrict__ list,
-                                                     uint32_t* __restrict__ count) {
-    const uint32_t n = *count;
+// Zero the accumulation grid of the static and dynamic lists; keep only the
+// static bit.
+__global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* __restrict__ dlist,
+                                                     const uint32_t* __restrict__ dcount) {
+    const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < n; i += gridDim.x * 4) {
-        const uint32_t t = list[i];
-        gmp[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane == 0) tile_flag[t] = 0u;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+        g.gmp[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 0) g.tile_flag[t] &= FLAG_STATIC;
     }
 }
 
-// Resets the counter after every block has read it (separate tiny launch).
+// Resets a counter after every block of the previous launch has read it.
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
-hipError_t launch_clear_tiles(float4* gmp, uint32_t* tile_flag, const uint32_t* list,
-                              uint32_t* count, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, gmp, tile_flag, list, count);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, count, 0u);
+hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, g, dlist, dcount);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, dcount, 0u);
+    return hipGetLastError();
+}
+
+// ---- static active list of a sort ----
+__global__ void k_static_clear(uint32_t* __restrict__ flag, const uint32_t* __restrict__ slist,
+                               const uint32_t* __restrict__ scount) {
+    const uint32_t n = *scount;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        flag[slist[i]] &= ~FLAG_STATIC;
+}
+
+// Every tile within one tile of an occupied tile (in each axis) may receive
+// P2G mass before the next sort.
+__global__ void k_static_mark(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                              const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flag,
+                              uint32_t* __restrict__ slist, uint32_t* __restrict__ scount) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t t = keys[j];
+        if (j != 0 && keys[j - 1] == t) continue;
+        const SceneDev sc = scenes[tile_scene[t]];
+        const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dxx = -1; dxx <= 1; ++dxx) {
+                    const int nx = tx + dxx, ny = ty + dy, nz = tz + dz;
+                    if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
+                    const uint32_t u = sc.tile_base + uint32_t(nx + W * (ny + W * nz));
+                    if (flag[u] & FLAG_STATIC) continue;
+                    if ((atomicOr(&flag[u], FLAG_STATIC) & FLAG_STATIC) == 0u) slist[atomicAdd(scount, 1u)] = u;
+                }
+    }
+}
+
+// tile_lb[k] = first sorted slot whose key is >= k, for k in [0, total].
+__global__ void k_tile_lb(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ lb,
+                          uint32_t total) {
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= total; k += gridDim.x * blockDim.x) {
+        uint32_t first = 0, last = n;
+        while (first < last) {
+            const uint32_t mid = first + (last - first) / 2;
+            if (keys[mid] < k) first = mid + 1;
+            else last = mid;
+        }
+        lb[k] = first;
+    }
+}
+
+hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_scene,
+                                 uint32_t* tile_flag, uint32_t* slist, uint32_t* scount,
+                                 const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_lb,
+                                 uint32_t total_tiles, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_static_clear, dim3(blocks), dim3(256), 0, s, tile_flag, slist, scount);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, scount, 0u);
+    if (n > 0) {
+        unsigned nb = std::min<unsigned>((n + 255) / 256, 8192u);
+        hipLaunchKernelGGL(k_static_mark, dim3(nb), dim3(256), 0, s, scenes, tile_scene, sorted_keys, n, tile_flag,
+                           slist, scount);
+    }
+    unsigned lbb = std::min<unsigned>((total_tiles + 256) / 256, 16384u);
+    hipLaunchKernelGGL(k_tile_lb, dim3(lbb), dim3(256), 0, s, sorted_keys, n, tile_lb, total_tiles);
     return hipGetLastError();
 }
 
@@ -675,23 +808,35 @@ hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t
     return hipGetLastError();
 }
 
-// dst slot j <- src slot slot_of_id[sorted_ids[j]], all planes.
-__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ src, float* __restrict__ dst,
-                                                 uint32_t cap, const uint32_t* __restrict__ sorted_ids,
-                                                 const uint32_t* __restrict__ slot_of_id, uint32_t n) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t old = slot_of_id[sorted_ids[j]];
-#pragma unroll 6
-        for (int k = 0; k < NCOMP; ++k) dst[(size_t)k * cap + j] = src[(size_t)k * cap + old];
-    }
+// src_slot[j] = slot_of_id[sorted_ids[j]]; then every plane is gathered with
+// one (plane, slot) pair per thread so reads and writes stay coalesced.
+__global__ void k_reorder_index(const uint32_t* __restrict__ sorted_ids, const uint32_t* __restrict__ slot_of_id,
+                                uint32_t* __restrict__ src_slot, uint32_t n) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        src_slot[j] = slot_of_id[sorted_ids[j]];
 }
 
-hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* sorted_ids,
-                          const uint32_t* slot_of_id, uint32_t n, hipStream_t s) {
+hipError_t launch_reorder_index(const uint32_t* sorted_ids, const uint32_t* slot_of_id,
+                                uint32_t* src_slot, uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    unsigned blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, src, dst, cap, sorted_ids, slot_of_id, n);
+    unsigned blocks = std::min<unsigned>((n + 255) / 256, 8192u);
+    hipLaunchKernelGGL(k_reorder_index, dim3(blocks), dim3(256), 0, s, sorted_ids, slot_of_id, src_slot, n);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ src, float* __restrict__ dst,
+                                                 uint32_t cap, const uint32_t* __restrict__ src_slot,
+                                                 uint32_t n) {
+    const size_t plane = (size_t)blockIdx.y * cap;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        dst[plane + j] = src[plane + src_slot[j]];
+}
+
+hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
+                          uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = std::min<unsigned>((n + 255) / 256, 2048u);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks, NCOMP), dim3(256), 0, s, src, dst, cap, src_slot, n);
     return hipGetLastError();
 }
 
@@ -770,56 +915,6 @@ hipError_t launch_grid_to_dense(const float4* gmp, const SceneDev* scenes, uint3
 
 // ---------------------------------------------------------------------------
 // Deterministic gather-order P2G (mpm_deterministic_p2g.wgsl:12-150).
-__global__ void k_tile_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ tb,
-                              uint32_t* __restrict__ te) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t k = keys[j];
-        if (j == 0 || keys[j - 1] != k) tb[k] = j;
-        if (j + 1 == n || keys[j + 1] != k) te[k] = j + 1;
-    }
-}
-
-hipError_t launch_tile_ranges(const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_begin,
-                              uint32_t* tile_end, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    unsigned blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_tile_ranges, dim3(blocks), dim3(256), 0, s, sorted_keys, n, tile_begin, tile_end);
-    return hipGetLastError();
-}
-
-// Every tile within one tile of an occupied tile may receive mass.
-__global__ void k_mark_dilated(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
-                               const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flag,
-                               uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t t = keys[j];
-        if (j != 0 && keys[j - 1] == t) continue;
-        const SceneDev sc = scenes[tile_scene[t]];
-        const uint32_t tl = t - sc.tile_base;
-        const int W = int(sc.W);
-        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dxx = -1; dxx <= 1; ++dxx) {
-                    int nx = tx + dxx, ny = ty + dy, nz = tz + dz;
-                    if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
-                    mark_tile(flag, list, count, sc.tile_base + uint32_t(nx + W * (ny + W * nz)));
-                }
-    }
-}
-
-hipError_t launch_mark_dilated(const SceneDev* scenes, const uint32_t* tile_scene,
-                               const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_flag,
-                               uint32_t* list, uint32_t* count, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    unsigned blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_mark_dilated, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, sorted_keys, n,
-                       tile_flag, list, count);
-    return hipGetLastError();
-}
-
 // Per particle: affine matrix A (planes 0-8) and m v (planes 9-11).
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_gather_prepare(StepArgs a, float* __restrict__ aff) {
@@ -868,8 +963,7 @@ hipError_t launch_gather_prepare(const StepArgs& a, float* affine, int blocks, h
 // ascending sorted slot (mpm_deterministic_p2g.wgsl:100-144): a fixed order,
 // so the result is bitwise repeatable.
 __global__ __launch_bounds__(64) void k_gather_p2g(StepArgs a, const uint32_t* __restrict__ tile_scene,
-                                                   const uint32_t* __restrict__ tb,
-                                                   const uint32_t* __restrict__ te,
+                                                   const uint32_t* __restrict__ lb,
                                                    const float* __restrict__ aff,
                                                    const uint32_t* __restrict__ list,
                                                    const uint32_t* __restrict__ count) {
@@ -888,7 +982,7 @@ __global__ __launch_bounds__(64) void k_gather_p2g(StepArgs a, const uint32_t* _
             const int nx = tx + ti % 3 - 1, ny = ty + (ti / 3) % 3 - 1, nz = tz + ti / 9 - 1;
             if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
             const uint32_t key = sc.tile_base + uint32_t(nx + W * (ny + W * nz));
-            const uint32_t jb = tb[key], je = te[key];
+            const uint32_t jb = lb[key], je = lb[key + 1];
             for (uint32_t j = jb; j < je; ++j) {
                 float xp[3], fx[3];
                 int off[3];
@@ -920,15 +1014,14 @@ __global__ __launch_bounds__(64) void k_gather_p2g(StepArgs a, const uint32_t* _
         }
         const bool valid = cx < int(sc.G) && cy < int(sc.G) && cz < int(sc.G);
         a.gmp[(t << 6) | uint32_t(lane)] = valid ? make_float4(ms, p0, p1, p2) : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane == 0) a.tile_flag[t] |= FLAG_GMP;
     }
 }
 
-hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene,
-                             const uint32_t* tile_begin, const uint32_t* tile_end,
-                             const float* affine, const uint32_t* list, const uint32_t* count,
+hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene, const uint32_t* tile_lb,
+                             const float* affine, const uint32_t* slist, const uint32_t* scount,
                              int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_gather_p2g, dim3(blocks), dim3(64), 0, s, a, tile_scene, tile_begin, tile_end,
-                       affine, list, count);
+    hipLaunchKernelGGL(k_gather_p2g, dim3(blocks), dim3(64), 0, s, a, tile_scene, tile_lb, affine, slist, scount);
     return hipGetLastError();
 }
 

@@ -1,0 +1,181 @@
+// mpm_scan.hip -- device-wide exclusive scan of u32 and the device-built
+// chunk list of the fused particle pass.
+//
+// Chunks (one block of the particle pass each) are cut from the sorted
+// particle array so that a chunk never leaves its *segment*: one tile row of
+// one scene, four consecutive tiles along x.  A chunk's particles then stay
+// inside a 4x1x1-tile region (plus motion since the sort), so its P2G box
+// always fits the block's LDS budget.
+#include <algorithm>
+
+#include "mpm_internal.hpp"
+
+namespace mpm {
+
+constexpr int SCAN_THREADS = 1024;
+constexpr int SCAN_ITEMS = 4;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+size_t scan_scratch_words(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + SCAN_THREADS + 2; }
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_warp, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_warp[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        const int nw = blockDim.x >> 6;
+        uint32_t w = lane < nw ? s_warp[lane] : 0u;
+        uint32_t wi = w;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t t = (uint32_t)__shfl_up((int)wi, o, 64);
+            if (lane >= o) wi += t;
+        }
+        if (lane < nw) s_warp[lane] = wi - w;
+        if (lane == nw - 1) s_warp[63] = wi;
+    }
+    __syncthreads();
+    const uint32_t r = incl - v + s_warp[wave];
+    if (total) *total = s_warp[63];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const uint32_t* __restrict__ in, size_t n,
+                                                              uint32_t* __restrict__ sums) {
+    __shared__ uint32_t s_warp[64];
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t tot = 0;
+    block_excl_scan(s, s_warp, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// Exclusive scan of `nb` block sums in place by one block; total in sums[nb].
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nb) {
+    __shared__ uint32_t s_warp[64];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t off = 0; off < nb; off += SCAN_THREADS) {
+        const uint32_t i = off + threadIdx.x;
+        const uint32_t v = i < nb ? sums[i] : 0u;
+        uint32_t tot = 0;
+        const uint32_t e = block_excl_scan(v, s_warp, &tot);
+        const uint32_t c = carry;
+        if (i < nb) sums[i] = c + e;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[nb] = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const uint32_t* __restrict__ in, size_t n,
+                                                            const uint32_t* __restrict__ sums,
+                                                            uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_warp[64];
+    const size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t run = block_excl_scan(s, s_warp, nullptr) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+// out[i] = sum(in[0..i)); total (sum of all n) is left in scratch[nblocks].
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch,
+                              uint32_t** total, hipStream_t s) {
+    const uint32_t nb = uint32_t((n + SCAN_TILE - 1) / SCAN_TILE);
+    if (total) *total = scratch + nb;
+    if (n == 0) {
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, scratch, 0u);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, scratch);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, scratch, nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, scratch, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Chunk list of the sorted particle array.
+__device__ __forceinline__ void segment_of(const SceneDev& sc, uint32_t key, uint32_t& first, uint32_t& ntiles) {
+    const uint32_t tl = key - sc.tile_base;
+    const uint32_t tx = tl % sc.W;
+    const uint32_t g0 = tx & ~3u;
+    first = key - (tx - g0);
+    ntiles = min(4u, sc.W - g0);
+}
+
+__global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ lb, uint32_t n,
+                              uint32_t* __restrict__ flags) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t key = keys[j];
+        const SceneDev& sc = scenes[tile_scene[key]];
+        uint32_t first, nt;
+        segment_of(sc, key, first, nt);
+        const uint32_t seg_begin = lb[first];
+        flags[j] = ((j - seg_begin) % CHUNK_PARTICLES) == 0u ? 1u : 0u;
+    }
+}
+
+__global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                             const uint32_t* __restrict__ keys, const uint32_t* __restrict__ lb, uint32_t n,
+                             const uint32_t* __restrict__ flags, const uint32_t* __restrict__ excl,
+                             Chunk* __restrict__ chunks, uint32_t* __restrict__ chunk_of_slot) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t c = excl[j] + flags[j] - 1u;  // chunk holding slot j
+        chunk_of_slot[j] = c;
+        if (!flags[j]) continue;
+        const uint32_t key = keys[j];
+        const uint32_t scn = tile_scene[key];
+        const SceneDev& sc = scenes[scn];
+        uint32_t first, nt;
+        segment_of(sc, key, first, nt);
+        const uint32_t seg_end = lb[first + nt];
+        Chunk ch;
+        ch.scene = scn;
+        ch.start = j;
+        ch.count = min((uint32_t)CHUNK_PARTICLES, seg_end - j);
+        ch.pad = 0;
+        chunks[c] = ch;
+    }
+}
+
+__global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) { *dst = *src; }
+
+hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
+                               const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
+                               uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
+                               uint32_t* chunk_of_slot, hipStream_t s) {
+    const unsigned blocks = std::max(1u, std::min<unsigned>((n + 255) / 256, 8192u));
+    hipLaunchKernelGGL(k_chunk_flags, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, flags);
+    uint32_t* total = nullptr;
+    hipError_t e = exclusive_scan_u32(flags, excl, n, scan_scratch, &total, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_chunk_emit, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, flags, excl,
+                       chunks, chunk_of_slot);
+    hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, s, total, n_chunks);
+    return hipGetLastError();
+}
+
+}  // namespace mpm

@@ -5,7 +5,7 @@
 // order.  Only the key bits that can be non-zero are sorted (ceil(bits/8)
 // passes).  Each pass is three launches:
 //   k_hist    per-block digit histogram (blocks of 4096 elements, LDS atomics)
-//   k_scan    exclusive scan of the digit-major histogram (one block)
+//   scan      device-wide exclusive scan of the digit-major histogram
 //   k_scatter stable scatter: within a block, elements are ranked in index
 //             order with 64-lane ballots (8 ballots give the lanes that share
 //             a digit), per-wave digit counts combined in LDS.
@@ -21,8 +21,9 @@ constexpr int RADIX = 256;
 static inline uint32_t sort_blocks(uint32_t n) { return (n + SORT_TILE - 1) / SORT_TILE; }
 
 size_t radix_scratch_words(uint32_t n) {
-    // ping-pong keys + values, histogram (RADIX * blocks)
-    return 2 * (size_t)n + (size_t)RADIX * sort_blocks(n) + 64;
+    // ping-pong keys + values, histogram (RADIX * blocks), its scan scratch
+    const size_t h = (size_t)RADIX * sort_blocks(n);
+    return 2 * (size_t)n + 2 * h + scan_scratch_words(h) + 64;
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void k_hist(const uint32_t* __restrict__ keys, uint32_t n,
@@ -38,30 +39,6 @@ __global__ __launch_bounds__(SORT_THREADS) void k_hist(const uint32_t* __restric
     }
     __syncthreads();
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of `total` u32 in place by one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ data, uint32_t total) {
-    __shared__ uint32_t part[1024];
-    const uint32_t per = (total + 1023) / 1024;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = min(b + per, total);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += data[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = part[threadIdx.x] - s;
-    for (uint32_t i = b; i < e; ++i) {
-        uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
 }
 
 __global__ __launch_bounds__(SORT_THREADS) void k_scatter(const uint32_t* __restrict__ kin,
@@ -128,14 +105,17 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t n, int bits
     uint32_t* k2 = scratch;
     uint32_t* v2 = scratch + n;
     uint32_t* hist = scratch + 2 * (size_t)n;
+    uint32_t* offs = hist + (size_t)RADIX * nb;
+    uint32_t* sscr = offs + (size_t)RADIX * nb;
     const int passes = (bits + 7) / 8;
     uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_hist, dim3(nb), dim3(SORT_THREADS), 0, s, ka, n, shift, nb, hist);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist, (uint32_t)RADIX * nb);
+        hipError_t e = exclusive_scan_u32(hist, offs, (size_t)RADIX * nb, sscr, nullptr, s);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(SORT_THREADS), 0, s, ka, va, kb, vb, n, shift, nb,
-                           hist);
+                           offs);
         uint32_t* t = ka;
         ka = kb;
         kb = t;

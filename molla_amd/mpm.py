@@ -76,7 +76,8 @@ class SceneConfig(C.Structure):
 
 class StatusInfo(C.Structure):
     _fields_ = [("fault_bits", C.c_uint32), ("completed_gathers", C.c_uint32),
-                ("active_tiles", C.c_uint32), ("sorts", C.c_uint32)]
+                ("active_tiles", C.c_uint32), ("sorts", C.c_uint32), ("escapes", C.c_uint32),
+                ("fallback_chunks", C.c_uint32), ("chunks", C.c_uint32)]
 
 
 _lib = None
@@ -329,7 +330,8 @@ class MpmSimulation:
         self._check(lib().mpm_read_status(self._h, C.byref(st)))
         return {"fault_bits": st.fault_bits, "overflow": bool(st.fault_bits & 1),
                 "numerical_fault": bool(st.fault_bits & 2), "completed_gathers": st.completed_gathers,
-                "active_tiles": st.active_tiles, "sorts": st.sorts,
+                "active_tiles": st.active_tiles, "sorts": st.sorts, "escapes": st.escapes,
+                "fallback_chunks": st.fallback_chunks, "chunks": st.chunks,
                 "faulted": st.fault_bits != 0}
 
     read_active_grid_status = read_status
