@@ -83,12 +83,41 @@ struct Scatter {
     float A[9];  // m C + 4/dx^2 stress
 };
 
-// LDS workspace of the block scatter.
+// LDS workspace of the block scatter.  The block's box is accumulated as
+// 64-bit fixed point: on gfx950 LDS integer atomics run at about the LDS
+// read rate while LDS f32 atomic adds are ~20x slower, and integer sums are
+// independent of the order in which lanes and waves arrive.
 struct ScatterLds {
-    float acc[4][LDS_CELLS];  // (m, px, py, pz) planes of the block's box: 32 KiB
+    unsigned long long acc[4][LDS_CELLS];  // (m, px, py, pz): 64 KiB
     int red[WAVES * 6];
+    float redf[WAVES * 2];
     int box[16];
+    double scale[2];  // fixed-point scale of the mass and momentum channels
 };
+
+// round(v * s) as a two's-complement int64 for |v * s| < 2^51: adding
+// 1.5 * 2^52 leaves the rounded integer in the low mantissa bits.
+__device__ __forceinline__ unsigned long long fixed_point(float v, double s) {
+    const double t = (double)v * s + 6755399441055744.0;
+    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000ll);
+}
+
+__device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Smallest power of two >= |x| (x > 0), as its exponent.
+__device__ __forceinline__ int exp2_ceil(float x) {
+    int e;
+    const float m = frexpf(x, &e);  // x = m * 2^e, m in [0.5, 1)
+    return m == 0.5f ? e - 1 : e;
+}
+
+__device__ __forceinline__ float wave_maxf(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
 
 // One particle's 27-node P2G straight to global f32 atomics.
 __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev& sc, const Scatter& p) {
@@ -133,12 +162,24 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         lo[k] = wave_min(lds_on ? p.b[k] : 0x3fffffff);
         hi[k] = wave_max(lds_on ? p.b[k] + 2 : -0x3fffffff);
     }
+    // bounds on any single contribution: w <= 1 and |d| <= 1.5 dx
+    float bm = 0.f, bp = 0.f;
+    if (lds_on) {
+        bm = fabsf(p.m);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            bp = fmaxf(bp, fabsf(p.mv[r]) + 1.5f * dx * (fabsf(p.A[3 * r]) + fabsf(p.A[3 * r + 1]) + fabsf(p.A[3 * r + 2])));
+    }
+    bm = wave_maxf(bm);
+    bp = wave_maxf(bp);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             L.red[wave * 6 + k] = lo[k];
             L.red[wave * 6 + 3 + k] = hi[k];
         }
+        L.redf[wave * 2] = bm;
+        L.redf[wave * 2 + 1] = bp;
     }
     __syncthreads();
     if (tid == 0) {
@@ -154,6 +195,14 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             l[k] = max(l[k], 0);
             h[k] = min(h[k], int(sc.G) - 1);
         }
+        float mm = L.redf[0], mp = L.redf[1];
+        for (int w = 1; w < WAVES; ++w) {
+            mm = fmaxf(mm, L.redf[w * 2]);
+            mp = fmaxf(mp, L.redf[w * 2 + 1]);
+        }
+        // contributions stay below 2^50 in fixed point; 256 of them sum below 2^58
+        L.scale[0] = mm > 0.f && mm == mm ? ldexp(1.0, 50 - exp2_ceil(mm)) : 1.0;
+        L.scale[1] = mp > 0.f && mp == mp ? ldexp(1.0, 50 - exp2_ceil(mp)) : 1.0;
         const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
         int use = 2;  // nothing reduced in LDS
         if (bx > 0 && by > 0 && bz > 0)
@@ -177,11 +226,12 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
     if (use == 1) {
         const int x0 = L.box[0], y0 = L.box[1], z0 = L.box[2], bx = L.box[3], by = L.box[4];
         const int vol = bx * by * L.box[5];
+        const double sm = L.scale[0], sp = L.scale[1];
         for (int i = tid; i < vol; i += BLOCK) {
-            L.acc[0][i] = 0.0f;
-            L.acc[1][i] = 0.0f;
-            L.acc[2][i] = 0.0f;
-            L.acc[3][i] = 0.0f;
+            L.acc[0][i] = 0ull;
+            L.acc[1][i] = 0ull;
+            L.acc[2][i] = 0ull;
+            L.acc[3][i] = 0ull;
         }
         __syncthreads();
         if (lds_on) {
@@ -199,15 +249,20 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                         float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
                               d2 = (float(k) - p.fx[2]) * dx;
                         int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-                        lds_add(&L.acc[0][li], wt * p.m);
-                        lds_add(&L.acc[1][li], wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
-                        lds_add(&L.acc[2][li], wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
-                        lds_add(&L.acc[3][li], wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
+                        lds_add_u64(&L.acc[0][li], fixed_point(wt * p.m, sm));
+                        lds_add_u64(&L.acc[1][li], fixed_point(wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2), sp));
+                        lds_add_u64(&L.acc[2][li], fixed_point(wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2), sp));
+                        lds_add_u64(&L.acc[3][li], fixed_point(wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2), sp));
                     }
         }
         __syncthreads();
+        const double im = 1.0 / sm, ip = 1.0 / sp;
         float4* part = a.part + (size_t)chunk * LDS_CELLS;
-        for (int i = tid; i < vol; i += BLOCK) part[i] = make_float4(L.acc[0][i], L.acc[1][i], L.acc[2][i], L.acc[3][i]);
+        for (int i = tid; i < vol; i += BLOCK)
+            part[i] = make_float4((float)((double)(long long)L.acc[0][i] * im),
+                                  (float)((double)(long long)L.acc[1][i] * ip),
+                                  (float)((double)(long long)L.acc[2][i] * ip),
+                                  (float)((double)(long long)L.acc[3][i] * ip));
         __syncthreads();
     }
     if (p.on && (p.esc || use == 0)) {
