@@ -10,7 +10,10 @@ Prints ONE JSON line.  Besides the headline value it carries:
   roofline      the dominant kernel (fused G2P->P2G) against the HBM peak, with
                 algorithmic bytes = 192 B per particle-substep (f32 x, v, C, F
                 read once and written once) and its duration measured with HIP
-                events on the library's stream over the timed substeps
+                events recorded on the library's stream around every launch
+                inside the timed substeps; `traffic` (HBM bytes per launch) is
+                taken from the FETCH_SIZE/WRITE_SIZE counter summary in
+                profiles/, scaled by the same counters on a copy of known size
   cpu_baseline  the FP64 single-core oracle (restatement of the reference CPU
                 path) on a bounded sample of the same workload
   batched       the 512-scene batch (one launch per substep, hipGraph replay)
@@ -48,6 +51,45 @@ def build_large(sim_cls, scale):
     sim.write_field("volume", d["volume"])
     sim.write_field("material", d["material"])
     return sim, cfg, d, meta
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_particles_g2p2g_large_scene_traffic.json")
+
+
+def calibrate_copy(mib):
+    """Device-to-device copy of `mib` MiB (one blit kernel per call), for
+    scaling the HBM byte counters: run under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE next to the MPM kernels."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = int(mib) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {nbytes} B device-to-device")
+
+
+def measured_traffic(n):
+    """HBM bytes per G2P2G launch from the committed counter summary, if it was
+    taken on the same particle count."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("particles") != n:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def time_steps(sim, dt, k, graph):
@@ -96,14 +138,16 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the workload size")
-    ap.add_argument("--cpu-substeps", type=int, default=3)
+    ap.add_argument("--cpu-substeps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay step_n as a hipGraph")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a device copy of MIB MiB (for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(scenes shard across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+              "(the batched workload would shard by scene with no communication)", file=sys.stderr)
         sys.exit(2)
 
     from molla_amd import MpmSimulation
@@ -112,28 +156,31 @@ def main():
     n = len(d["x"])
     dt = meta["dt"]
     log(f"large_scene: {n} particles, grid {meta['grid']}^3, dt {dt}")
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
     if args.warmup:
         time_steps(sim, dt, args.warmup, args.graph)
-    el = time_steps(sim, dt, args.steps, args.graph)
-    value = n * args.steps / el
-    log(f"timed {args.steps} substeps: {el * 1e3:.2f} ms -> {value:.3e} particle-substeps/s")
-
-    # per-kernel durations over a second run of the same K substeps
-    sim.set_profiling(True)
+    # HIP events around every launch of the timed substeps (graph replays are
+    # one opaque launch, so per-kernel events need the eager path)
+    sim.set_profiling(not args.graph)
     sim.reset_profile()
-    sim.step_n(dt, args.steps)
-    sim.synchronize()
+    el = time_steps(sim, dt, args.steps, args.graph)
     prof = sim.read_profile()
     sim.set_profiling(False)
+    value = n * args.steps / el
+    log(f"timed {args.steps} substeps: {el * 1e3:.2f} ms -> {value:.3e} particle-substeps/s")
     st = sim.read_status()
     ms, cnt = prof["g2p2g"]
     avg = ms / max(cnt, 1) * 1e-3
-    achieved = BYTES_FWD * n / avg / 1e9 if avg > 0 else 0.0
+    achieved = BYTES_FWD * n / avg / 1e9 if avg > 0 else None
+    traffic = measured_traffic(n)
     roofline = {"bound": "hbm", "kernel": "k_particles<G2P2G>", "achieved": achieved,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": None, "avg_launch_ms": avg * 1e3,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                "traffic": traffic, "avg_launch_ms": avg * 1e3, "launches": cnt,
                 "algorithmic_bytes_per_launch": BYTES_FWD * n,
-                "stage_ms_per_substep": {k: (v[0] / max(v[1], 1)) for k, v in prof.items()}}
+                "stage_ms_per_substep": {k: (v[0] / args.steps) for k, v in prof.items()},
+                "stage_launches": {k: v[1] for k, v in prof.items()}}
     log(f"profile: {prof}")
     log(f"status: {st}")
 

@@ -38,7 +38,7 @@ typedef enum {
     MPM_ERR_FAULT = 3    /* a latched device fault blocks the request  */
 } mpm_status;
 
-/* Semantics (see DESIGN.md): the GPU step of mpm.wgsl, or the differentiable
+/* Semantics: the GPU step of mpm.wgsl, or the differentiable
  * CPU step of molla-autograd/src/mpm_cpu_diff.rs. */
 typedef enum { MPM_MODEL_SHADER = 0, MPM_MODEL_CPUDIFF = 1 } mpm_model;
 

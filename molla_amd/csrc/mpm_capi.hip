@@ -414,7 +414,7 @@ int deterministic_p2g(mpm_sim* s, float dt) {
     return MPM_OK;
 }
 
-// The substep sequence of one step_n call (see DESIGN.md "Stepping").
+// The substep sequence of one step_n call.
 int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
     int rc;
     if (!s->dry && (rc = ensure_grid(s)) != MPM_OK) return rc;

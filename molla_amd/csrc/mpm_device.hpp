@@ -141,11 +141,10 @@ __device__ __forceinline__ int argmin3(double a, double b, double c) {
     return k;
 }
 
-// model 0: shader convention; model 1: CPU convention.  Output in f32.
+// Sign fix of svd3_f64_raw's output.  model 0: shader convention; model 1:
+// CPU convention.
 template <int MODEL>
-__device__ __forceinline__ void svd3(const float* F, Svd32& o) {
-    double U[9], s[3], V[9];
-    svd3_f64_raw(F, U, s, V);
+__device__ __forceinline__ void svd3_signs(double* U, double* s, double* V) {
     if (det3d(U) < 0.0) {
         int k = argmin3(s[0], s[1], s[2]);
         U[k] = -U[k];
@@ -159,6 +158,14 @@ __device__ __forceinline__ void svd3(const float* F, Svd32& o) {
             V[6 + k] = -V[6 + k];
         }
     }
+}
+
+// FP64 SVD with the sign convention of MODEL, rounded to f32.
+template <int MODEL>
+__device__ __forceinline__ void svd3(const float* F, Svd32& o) {
+    double U[9], s[3], V[9];
+    svd3_f64_raw(F, U, s, V);
+    svd3_signs<MODEL>(U, s, V);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         o.U[i] = (float)U[i];

@@ -136,7 +136,7 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
                               const uint32_t* dcount, uint32_t* next_count, float dt,
                               int zero_after, int use_partials, int blocks, hipStream_t s);
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
-                              hipStream_t s);
+                              hipStream_t s, float4* also = nullptr);
 // Static active list of a sort: the 3x3x3 dilation of occupied tiles.
 hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_scene,
                                  uint32_t* tile_flag, uint32_t* slist, uint32_t* scount,
@@ -162,6 +162,26 @@ hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene, cons
                              int blocks, hipStream_t s);
 hipError_t launch_reorder_index(const uint32_t* sorted_ids, const uint32_t* slot_of_id,
                                 uint32_t* src_slot, uint32_t n, hipStream_t s);
+
+// ---- reverse pass of the CPUDIFF substep (mpm_diff.hip) ----
+// Adjoint particle planes (same slot order and plane stride as the state).
+enum AdjComp : int { AX = 0, AV = 3, AC = 6, AF = 15, AE = 24, NADJ = 25 };
+
+struct AdjArgs {
+    StepArgs a;    // a.ps: the substep's pre-state; a.gmp: atomic target of the reverse G2P scatter
+    float* adj;    // NADJ planes of a.cap floats
+    float4* gadj;  // per cell: (dJ/dm, dJ/dp) after the grid reverse
+    double* gml;   // per chunk: accumulated (dJ/dmu, dJ/dlambda)
+};
+// Reverse G2P + x/F advance: gathers grid v, scatters dJ/d(grid v) (partials),
+// folds the G2P part of dJ/dx into adj x and replaces adj F by (I + dt C)^T adj F.
+hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s);
+// Grid reverse over the active tiles: dJ/d(grid v) -> (dJ/dm, dJ/dp) in gadj.
+hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
+                           const uint32_t* dcount, float dt, int blocks, hipStream_t s);
+// Reverse P2G + stress: gathers gadj; writes adj x, v, C, F of the pre-state
+// and accumulates adj E and the per-chunk (mu, lambda) sums.
+hipError_t launch_adj_p2g(const AdjArgs& q, int blocks, hipStream_t s);
 
 // ---- scans and the sorted chunk list (mpm_scan.hip) ----
 size_t scan_scratch_words(size_t n);

@@ -7,269 +7,19 @@
 //                  p2g/p2g_two_pass :494-925) so a particle's state is read
 //                  once and written once per substep and the polar/SVD of F is
 //                  computed once per substep (FP64).
-// The P2G half stages the block's scatter in LDS over the bounding box of its
-// particles' stencils (particles are kept sorted by 4^3 tile, so that box is
-// small), then flushes each touched cell with one global f32 atomic per
-// channel and appends newly touched tiles to the next active list.
+// The P2G half reduces the block's scatter in LDS (64-bit fixed point) over
+// the bounding box of its particles' stencils -- particles are kept sorted by
+// 4^3 tile and cut into chunks that stay inside four tiles of one tile row,
+// so that box is small -- and stores the box as the chunk's partial with
+// plain stores.  The next grid update sums the partials of the chunks around
+// each active tile in ascending chunk order.  Particles that moved beyond the
+// 27 tiles around their sorted tile since the last sort scatter with global
+// f32 atomics instead.
 #include <algorithm>
 
-#include "mpm_device.hpp"
-#include "mpm_internal.hpp"
+#include "mpm_scatter.hpp"
 
 namespace mpm {
-
-constexpr int BLOCK = 256;
-constexpr int WAVES = BLOCK / 64;
-
-__device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int iy, int iz) {
-    uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
-    return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
-}
-
-__device__ __forceinline__ bool in_grid(const SceneDev& sc, int ix, int iy, int iz) {
-    int n = int(sc.G);
-    return ix >= 0 && iy >= 0 && iz >= 0 && ix < n && iy < n && iz < n;
-}
-
-__device__ __forceinline__ void lds_add(float* p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void global_add(float* p, float v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void mark_tile(uint32_t* flag, uint32_t* list, uint32_t* count, uint32_t t) {
-    if ((__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & (FLAG_DYN | FLAG_STATIC)) == 0u) {
-        if ((atomicOr(&flag[t], FLAG_DYN) & (FLAG_DYN | FLAG_STATIC)) == 0u) {
-            uint32_t slot = atomicAdd(count, 1u);
-            list[slot] = t;
-        }
-    }
-}
-
-__device__ __forceinline__ void mark_gmp(uint32_t* flag, uint32_t t) {
-    if ((__hip_atomic_load(&flag[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_GMP) == 0u)
-        atomicOr(&flag[t], FLAG_GMP);
-}
-
-__device__ __forceinline__ void latch(uint32_t* status, uint32_t bits) {
-    atomicOr(&status[ST_FAULT], bits);
-}
-
-__device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Material row of a particle (mpm.wgsl:152-157): global table at scene base.
-__device__ __forceinline__ const float* mat_row(const StepArgs& a, const SceneDev& sc, uint32_t m) {
-    if (m >= sc.n_mat) m = 0;
-    return a.mats + (size_t)(sc.mat_base + m) * MAT_STRIDE;
-}
-
-struct Scatter {
-    bool on;   // the particle scatters this pass
-    bool esc;  // ... with global atomics (stencil left its sort neighbourhood)
-    int b[3];
-    float fx[3];
-    float m;
-    float mv[3];
-    float A[9];  // m C + 4/dx^2 stress
-};
-
-// LDS workspace of the block scatter.  The block's box is accumulated as
-// 64-bit fixed point: on gfx950 LDS integer atomics run at about the LDS
-// read rate while LDS f32 atomic adds are ~20x slower, and integer sums are
-// independent of the order in which lanes and waves arrive.
-struct ScatterLds {
-    unsigned long long acc[4][LDS_CELLS];  // (m, px, py, pz): 64 KiB
-    int red[WAVES * 6];
-    float redf[WAVES * 2];
-    int box[16];
-    double scale[2];  // fixed-point scale of the mass and momentum channels
-};
-
-// round(v * s) as a two's-complement int64 for |v * s| < 2^51: adding
-// 1.5 * 2^52 leaves the rounded integer in the low mantissa bits.
-__device__ __forceinline__ unsigned long long fixed_point(float v, double s) {
-    const double t = (double)v * s + 6755399441055744.0;
-    return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000ll);
-}
-
-__device__ __forceinline__ void lds_add_u64(unsigned long long* p, unsigned long long v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Smallest power of two >= |x| (x > 0), as its exponent.
-__device__ __forceinline__ int exp2_ceil(float x) {
-    int e;
-    const float m = frexpf(x, &e);  // x = m * 2^e, m in [0.5, 1)
-    return m == 0.5f ? e - 1 : e;
-}
-
-__device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// One particle's 27-node P2G straight to global f32 atomics.
-__device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev& sc, const Scatter& p) {
-    float w[3][3];
-    quad_weights(p.fx[0], w[0]);
-    quad_weights(p.fx[1], w[1]);
-    quad_weights(p.fx[2], w[2]);
-    const float dx = sc.dx;
-    for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < 3; ++j)
-            for (int i = 0; i < 3; ++i) {
-                int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
-                if (!in_grid(sc, ix, iy, iz)) continue;
-                float wt = w[0][i] * w[1][j] * w[2][k];
-                float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
-                      d2 = (float(k) - p.fx[2]) * dx;
-                uint32_t c = cell_index(sc, ix, iy, iz);
-                float* g = reinterpret_cast<float*>(&a.gmp[c]);
-                global_add(g + 0, wt * p.m);
-                global_add(g + 1, wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
-                global_add(g + 2, wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
-                global_add(g + 3, wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
-                mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
-                mark_gmp(a.tile_flag, c >> 6);
-            }
-}
-
-// P2G of one block's particles (chunk `chunk`).  Every thread of the block
-// must call this.  Contributions are reduced with LDS f32 atomics over the
-// bounding box of the block's stencils; the box is stored as the chunk's
-// partial with plain stores (no global atomics, no registration: the grid
-// update finds the chunk through the sort).  Escaped particles, and blocks
-// whose box exceeds LDS_CELLS, scatter with global atomics.
-__device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
-                                              uint32_t chunk, ScatterLds& L) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const float dx = sc.dx;
-    const bool lds_on = p.on && !p.esc;
-    int lo[3], hi[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = wave_min(lds_on ? p.b[k] : 0x3fffffff);
-        hi[k] = wave_max(lds_on ? p.b[k] + 2 : -0x3fffffff);
-    }
-    // bounds on any single contribution: w <= 1 and |d| <= 1.5 dx
-    float bm = 0.f, bp = 0.f;
-    if (lds_on) {
-        bm = fabsf(p.m);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-            bp = fmaxf(bp, fabsf(p.mv[r]) + 1.5f * dx * (fabsf(p.A[3 * r]) + fabsf(p.A[3 * r + 1]) + fabsf(p.A[3 * r + 2])));
-    }
-    bm = wave_maxf(bm);
-    bp = wave_maxf(bp);
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            L.red[wave * 6 + k] = lo[k];
-            L.red[wave * 6 + 3 + k] = hi[k];
-        }
-        L.redf[wave * 2] = bm;
-        L.redf[wave * 2 + 1] = bp;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        int l[3], h[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            l[k] = L.red[k];
-            h[k] = L.red[3 + k];
-            for (int w = 1; w < WAVES; ++w) {
-                l[k] = min(l[k], L.red[w * 6 + k]);
-                h[k] = max(h[k], L.red[w * 6 + 3 + k]);
-            }
-            l[k] = max(l[k], 0);
-            h[k] = min(h[k], int(sc.G) - 1);
-        }
-        float mm = L.redf[0], mp = L.redf[1];
-        for (int w = 1; w < WAVES; ++w) {
-            mm = fmaxf(mm, L.redf[w * 2]);
-            mp = fmaxf(mp, L.redf[w * 2 + 1]);
-        }
-        // contributions stay below 2^50 in fixed point; 256 of them sum below 2^58
-        L.scale[0] = mm > 0.f && mm == mm ? ldexp(1.0, 50 - exp2_ceil(mm)) : 1.0;
-        L.scale[1] = mp > 0.f && mp == mp ? ldexp(1.0, 50 - exp2_ceil(mp)) : 1.0;
-        const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
-        int use = 2;  // nothing reduced in LDS
-        if (bx > 0 && by > 0 && bz > 0)
-            use = (a.use_lds && bx * by * bz <= LDS_CELLS && chunk < a.part_capacity) ? 1 : 0;
-        L.box[0] = l[0];
-        L.box[1] = l[1];
-        L.box[2] = l[2];
-        L.box[3] = bx;
-        L.box[4] = by;
-        L.box[5] = bz;
-        L.box[6] = use;
-        if (use == 0 && a.use_lds) atomicAdd(&a.status[ST_FALLBACK], 1u);
-        if (a.use_lds && chunk < a.part_capacity) {
-            a.chunk_box[2 * (size_t)chunk] = make_int4(l[0], l[1], l[2], 0);
-            a.chunk_box[2 * (size_t)chunk + 1] =
-                use == 1 ? make_int4(bx, by, bz, bx * by * bz) : make_int4(0, 0, 0, 0);
-        }
-    }
-    __syncthreads();
-    const int use = L.box[6];
-    if (use == 1) {
-        const int x0 = L.box[0], y0 = L.box[1], z0 = L.box[2], bx = L.box[3], by = L.box[4];
-        const int vol = bx * by * L.box[5];
-        const double sm = L.scale[0], sp = L.scale[1];
-        for (int i = tid; i < vol; i += BLOCK) {
-            L.acc[0][i] = 0ull;
-            L.acc[1][i] = 0ull;
-            L.acc[2][i] = 0ull;
-            L.acc[3][i] = 0ull;
-        }
-        __syncthreads();
-        if (lds_on) {
-            float w[3][3];
-            quad_weights(p.fx[0], w[0]);
-            quad_weights(p.fx[1], w[1]);
-            quad_weights(p.fx[2], w[2]);
-            for (int k = 0; k < 3; ++k)
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
-                        if (!in_grid(sc, ix, iy, iz)) continue;
-                        float wt = w[0][i] * w[1][j] * w[2][k];
-                        float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
-                              d2 = (float(k) - p.fx[2]) * dx;
-                        int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-                        lds_add_u64(&L.acc[0][li], fixed_point(wt * p.m, sm));
-                        lds_add_u64(&L.acc[1][li], fixed_point(wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2), sp));
-                        lds_add_u64(&L.acc[2][li], fixed_point(wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2), sp));
-                        lds_add_u64(&L.acc[3][li], fixed_point(wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2), sp));
-                    }
-        }
-        __syncthreads();
-        const double im = 1.0 / sm, ip = 1.0 / sp;
-        float4* part = a.part + (size_t)chunk * LDS_CELLS;
-        for (int i = tid; i < vol; i += BLOCK)
-            part[i] = make_float4((float)((double)(long long)L.acc[0][i] * im),
-                                  (float)((double)(long long)L.acc[1][i] * ip),
-                                  (float)((double)(long long)L.acc[2][i] * ip),
-                                  (float)((double)(long long)L.acc[3][i] * ip));
-        __syncthreads();
-    }
-    if (p.on && (p.esc || use == 0)) {
-        if (p.esc) atomicAdd(&a.status[ST_ESCAPES], 1u);
-        scatter_global(a, sc, p);
-    }
-}
 
 // Kirchhoff-form stress of a particle from its SVD; returns the affine
 // matrix A = m C + 4/dx^2 * (-dt vol U diag(tau) U^T).
@@ -587,14 +337,7 @@ __global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
                 sp.m = m;
                 sp.on = true;
                 sp.esc = !a.use_lds;
-                if (a.use_lds) {
-                    // stencil must stay inside the 3x3x3 tiles around the sorted tile
-                    const uint32_t tl = a.sorted_keys[slot] - sc.tile_base;
-                    const int tk[3] = {int(tl % sc.W), int((tl / sc.W) % sc.W), int(tl / (sc.W * sc.W))};
-#pragma unroll
-                    for (int k = 0; k < 3; ++k)
-                        if (sp.b[k] < 4 * tk[k] - 4 || sp.b[k] > 4 * tk[k] + 5) sp.esc = true;
-                }
+                if (a.use_lds && stencil_escapes(a, sc, slot, sp.b)) sp.esc = true;
             }
         }
         if (MODE != MODE_G2P) block_scatter(a, sc, sp, ci, L);
@@ -675,43 +418,12 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
         const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
         const SceneDev sc = scenes[g.tile_scene[t]];
-        const uint32_t tl = t - sc.tile_base;
-        const int W = int(sc.W);
-        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
-        const int ix = tx * 4 + (lane & 3), iy = ty * 4 + ((lane >> 2) & 3), iz = tz * 4 + (lane >> 4);
+        int tx, ty, tz, ix, iy, iz;
+        tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (t << 6) | uint32_t(lane);
         const uint32_t flag = g.tile_flag[t];
         float4 q = (flag & FLAG_GMP) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (use_partials) {
-            int last = -1;
-            const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
-            for (int dz = -1; dz <= 1; ++dz) {
-                const int nz = tz + dz;
-                if (nz < 0 || nz >= W) continue;
-                for (int dy = -1; dy <= 1; ++dy) {
-                    const int ny = ty + dy;
-                    if (ny < 0 || ny >= W) continue;
-                    const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));
-                    const uint32_t s0 = g.tile_lb[row + xa], s1 = g.tile_lb[row + xb + 1];
-                    if (s0 >= s1) continue;
-                    const int c0 = max(int(g.chunk_of_slot[s0]), last + 1);
-                    const int c1 = min(int(g.chunk_of_slot[s1 - 1]), int(g.part_capacity) - 1);
-                    for (int chn = c0; chn <= c1; ++chn) {
-                        const int4 o = g.chunk_box[2 * (size_t)chn];
-                        const int4 d = g.chunk_box[2 * (size_t)chn + 1];
-                        const int lx = ix - o.x, ly = iy - o.y, lz = iz - o.z;
-                        if (lx >= 0 && ly >= 0 && lz >= 0 && lx < d.x && ly < d.y && lz < d.z) {
-                            const float4 r = g.part[(size_t)chn * LDS_CELLS + lx + d.x * (ly + d.y * lz)];
-                            q.x += r.x;
-                            q.y += r.y;
-                            q.z += r.z;
-                            q.w += r.w;
-                        }
-                    }
-                    last = max(last, c1);
-                }
-            }
-        }
+        if (use_partials) add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
         float4 out;
         cell_update(sc, q, ix, iy, iz, dt, out);
         g.gv[c] = out;
@@ -736,13 +448,15 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
 
 // Zero the accumulation grid of the static and dynamic lists; keep only the
 // static bit.
+// `also` (may be null) is a second per-cell buffer cleared on the same tiles.
 __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* __restrict__ dlist,
-                                                     const uint32_t* __restrict__ dcount) {
+                                                     const uint32_t* __restrict__ dcount, float4* also) {
     const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
         const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
         g.gmp[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (also) also[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (lane == 0) g.tile_flag[t] &= FLAG_STATIC;
     }
 }
@@ -751,8 +465,8 @@ __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, g, dlist, dcount);
+                              hipStream_t s, float4* also) {
+    hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, g, dlist, dcount, also);
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, dcount, 0u);
     return hipGetLastError();
 }
