@@ -199,6 +199,10 @@ int mpm_diff_forward(mpm_sim* sim, float dt, uint32_t steps, uint32_t checkpoint
  * (mpm_adjoint.rs:76-113). */
 int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad_v_final,
                       float* grad_x0, float* grad_v0, float* grad_E, double* grad_mu_lambda);
+/* The rest of the StateAdjoint of the last backward (mpm_adjoint.rs:76-113):
+ * dJ/d(initial field) in original particle order for field POSITION,
+ * VELOCITY, C, F (row-major 3x3) or YOUNGS. */
+int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n);
 
 #ifdef __cplusplus
 }

@@ -106,9 +106,18 @@ struct mpm_sim {
     bool capturing = false;
     bool dry = false;  // walk the schedule's host bookkeeping without launching
     // differentiable path
+    bool hold_order = false;   // no re-sort: the diff tape keeps one slot order
     std::vector<float*> ckpt;  // checkpoints of the 24 state planes
     uint32_t diff_steps = 0, diff_every = 0;
     float diff_dt = 0.f;
+    uint32_t diff_sorts = 0;
+    int diff_ps = 0;
+    float* tape = nullptr;     // pre-states of one checkpoint segment (NCOMP planes each)
+    uint32_t tape_len = 0;
+    float* adj = nullptr;      // NADJ adjoint planes
+    float4* gadj = nullptr;    // per-cell grid adjoint
+    double* gml = nullptr;     // per-chunk (dJ/dmu, dJ/dlambda)
+    bool adj_valid = false;
     std::string err;
 };
 
@@ -195,6 +204,9 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
+    void* dbufs[] = {s->tape, s->adj, s->gadj, s->gml};
+    for (void* b : dbufs)
+        if (b) hipFree(b);
     if (s->stream) hipStreamDestroy(s->stream);
 }
 
@@ -375,18 +387,23 @@ int sort_particles(mpm_sim* s, bool reorder) {
     return MPM_OK;
 }
 
-int particle_pass(mpm_sim* s, int mode, float dt, int list_out) {
+// Sorted layout: the device-built chunk list, walked by a persistent grid.
+void use_sorted_chunks(mpm_sim* s, mpm::StepArgs& a, int& blocks) {
+    a.chunks = s->schunks;
+    a.n_chunks_dev = s->status + mpm::ST_NCHUNKS;
+    a.n_chunks = std::min<uint32_t>(s->max_schunks, s->n / 256 + s->n / 64 + s->n_scenes + 64);
+    blocks = s->pblocks ? int(s->pblocks) : int(std::min<uint32_t>(a.n_chunks, 4096u));
+}
+
+// `ps` (optional) runs the pass on another set of state planes of the same
+// slot order (the adjoint's recomputed pre-states).
+int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = nullptr) {
     if (s->dry) return MPM_OK;
     Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
     mpm::StepArgs a = args(s, dt, list_out);
+    if (ps) a.ps = ps;
     int blocks = particle_blocks(s);
-    if (s->p2g_mode == MPM_P2G_FUSED && s->sorted) {
-        // sorted layout: the device-built chunk list, walked by a persistent grid
-        a.chunks = s->schunks;
-        a.n_chunks_dev = s->status + mpm::ST_NCHUNKS;
-        a.n_chunks = std::min<uint32_t>(s->max_schunks, s->n / 256 + s->n / 64 + s->n_scenes + 64);
-        blocks = s->pblocks ? int(s->pblocks) : int(std::min<uint32_t>(a.n_chunks, 4096u));
-    }
+    if (s->p2g_mode == MPM_P2G_FUSED && s->sorted) use_sorted_chunks(s, a, blocks);
     HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, blocks, s->stream));
     return MPM_OK;
 }
@@ -433,7 +450,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
         s->grid_dirty = true;
         return MPM_OK;
     }
-    if (fused && (!s->sorted || (s->sort_every == 0) || s->since_sort >= s->sort_every)) {
+    if (fused && !s->hold_order && (!s->sorted || (s->sort_every == 0) || s->since_sort >= s->sort_every)) {
         if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
     }
     // prologue P2G of the current state
@@ -442,7 +459,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
         const bool last = k + 1 == substeps;
         if ((rc = grid_update(s, dt, !last)) != MPM_OK) return rc;
         if (fused) {
-            if (!last && s->sort_every > 0 && s->since_sort >= s->sort_every) {
+            if (!last && !s->hold_order && s->sort_every > 0 && s->since_sort >= s->sort_every) {
                 if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
             }
             if ((rc = particle_pass(s, last ? mpm::MODE_G2P : mpm::MODE_G2P2G, dt, 1 - s->list_cur)) != MPM_OK)
@@ -630,6 +647,195 @@ int set_particles(mpm_sim* s, const uint32_t* counts, const float* pos, const fl
         if (rc != MPM_OK) return rc;
     }
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Differentiable path (mpm_cpu_diff.rs forward_with_tape :397-418,
+// mpm_adjoint.rs backward :121-137).  The forward sorts once and then keeps
+// that slot order for the whole run, storing the 24 state planes every
+// `every` substeps.  The backward walks the segments last to first: it
+// recomputes the segment's pre-states into the tape from its checkpoint and
+// then reverses each substep over its recomputed grid.
+int diff_check_models(mpm_sim* s) {
+    for (uint32_t i = 0; i < s->n_scenes; ++i)
+        if (s->cfg[i].model != MPM_MODEL_CPUDIFF)
+            return fail(s, MPM_ERR_INVALID, "the differentiable path is defined for MPM_MODEL_CPUDIFF scenes");
+    return MPM_OK;
+}
+
+int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
+    if (!std::isfinite(dt) || !(dt > 0.0f)) return fail(s, MPM_ERR_INVALID, "dt must be finite and > 0");
+    if (steps == 0 || every == 0) return fail(s, MPM_ERR_INVALID, "steps and checkpoint_every must be >= 1");
+    if (s->n == 0) return fail(s, MPM_ERR_INVALID, "no particles");
+    int rc = diff_check_models(s);
+    if (rc != MPM_OK) return rc;
+    if ((rc = ensure_grid(s)) != MPM_OK) return rc;
+    const uint32_t nseg = (steps + every - 1) / every;
+    const size_t ck = (size_t)24 * s->cap;
+    while (s->ckpt.size() < nseg) {
+        float* c = nullptr;
+        HIP_TRY(s, dalloc(&c, ck));
+        s->ckpt.push_back(c);
+    }
+    const int mode = s->p2g_mode, wvc = s->write_vc;
+    s->p2g_mode = MPM_P2G_FUSED;
+    s->write_vc = 1;
+    s->diff_steps = 0;
+    s->adj_valid = false;
+    rc = clear_grid(s);
+    if (rc == MPM_OK) rc = sort_particles(s, true);
+    s->hold_order = true;
+    for (uint32_t seg = 0; rc == MPM_OK && seg < nseg; ++seg) {
+        const uint32_t len = std::min(every, steps - seg * every);
+        hipError_t e = hipMemcpyAsync(s->ckpt[seg], s->ps[s->ps_cur], ck * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) rc = hip_fail(s, e, "checkpoint copy");
+        if (rc == MPM_OK) rc = run_step_n(s, dt, len);
+    }
+    s->hold_order = false;
+    s->p2g_mode = mode;
+    s->write_vc = wvc;
+    if (rc != MPM_OK) return rc;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->diff_steps = steps;
+    s->diff_every = every;
+    s->diff_dt = dt;
+    s->diff_sorts = s->sorts;
+    s->diff_ps = s->ps_cur;
+    return MPM_OK;
+}
+
+mpm::AdjArgs adj_args(mpm_sim* s, float dt, int list, float* ps, int& blocks) {
+    mpm::AdjArgs q{};
+    q.a = args(s, dt, list);
+    q.a.ps = ps;
+    q.a.gmp = s->gadj;  // the reverse G2P's escaped particles accumulate here
+    q.a.use_lds = 1;
+    use_sorted_chunks(s, q.a, blocks);
+    q.adj = s->adj;
+    q.gadj = s->gadj;
+    q.gml = s->gml;
+    return q;
+}
+
+int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
+    if (s->diff_steps == 0) return fail(s, MPM_ERR_INVALID, "mpm_diff_forward must run first");
+    if (s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
+        return fail(s, MPM_ERR_INVALID, "the particle order changed since mpm_diff_forward");
+    if (!gx) return fail(s, MPM_ERR_INVALID, "grad_x_final must not be NULL");
+    const uint32_t cap = s->cap, every = s->diff_every, steps = s->diff_steps;
+    const size_t plane = cap;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    if (s->tape_len < every) {
+        if (s->tape) HIP_TRY(s, hipFree(s->tape));
+        s->tape = nullptr;
+        HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * plane));
+        s->tape_len = every;
+    }
+    if (!s->adj) HIP_TRY(s, dalloc(&s->adj, (size_t)mpm::NADJ * plane));
+    if (!s->gml) HIP_TRY(s, dalloc(&s->gml, 2 * (size_t)s->max_schunks));
+    if (!s->gadj) {
+        HIP_TRY(s, dalloc(&s->gadj, cells));
+        HIP_TRY(s, hipMemsetAsync(s->gadj, 0, cells * sizeof(float4), s->stream));
+    }
+    const int mode = s->p2g_mode;
+    s->p2g_mode = MPM_P2G_FUSED;
+    int rc = clear_grid(s);
+    if (rc != MPM_OK) {
+        s->p2g_mode = mode;
+        return rc;
+    }
+    uint32_t gathers = 0;
+    HIP_TRY(s, hipMemcpyAsync(&gathers, s->status + mpm::ST_GATHERS, 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->adj, 0, (size_t)mpm::NADJ * plane * 4, s->stream));
+    HIP_TRY(s, hipMemsetAsync(s->gml, 0, 2 * (size_t)s->max_schunks * sizeof(double), s->stream));
+    HIP_TRY(s, hipMemcpyAsync(s->staging, gx, (size_t)s->n * 12, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AX, 3, s->slot_of_id, s->staging, s->n, s->stream));
+    if (gv) {
+        HIP_TRY(s, hipMemcpyAsync(s->staging, gv, (size_t)s->n * 12, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AV, 3, s->slot_of_id, s->staging, s->n, s->stream));
+    }
+    const float dt = s->diff_dt;
+    const size_t st_words = (size_t)mpm::NCOMP * plane;
+    const mpm::GridDev g = griddev(s);
+    const int gb = grid_blocks(s);
+    int L = s->list_cur;
+    auto count = [&](int l) { return s->status + mpm::ST_COUNT0 + l; };
+    const uint32_t nseg = (steps + every - 1) / every;
+    for (int seg = int(nseg) - 1; seg >= 0 && rc == MPM_OK; --seg) {
+        const uint32_t len = std::min(every, steps - uint32_t(seg) * every);
+        float* t0 = s->tape;
+        HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
+        HIP_TRY(s, hipMemcpyAsync(t0 + 24 * plane, s->ps[s->ps_cur] + 24 * plane, (size_t)(mpm::NCOMP - 24) * plane * 4,
+                                  hipMemcpyDeviceToDevice, s->stream));
+        // recompute the pre-states of substeps 1 .. len-1 of the segment
+        for (uint32_t k = 0; k + 1 < len && rc == MPM_OK; ++k) {
+            float* tk = s->tape + k * st_words;
+            float* tk1 = tk + st_words;
+            HIP_TRY(s, hipMemcpyAsync(tk1, tk, st_words * 4, hipMemcpyDeviceToDevice, s->stream));
+            if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, tk1)) != MPM_OK) break;
+            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
+                                               s->stream));
+            if ((rc = particle_pass(s, mpm::MODE_G2P, dt, L, tk1)) != MPM_OK) break;
+            L = 1 - L;
+        }
+        // reverse the segment's substeps
+        for (int k = int(len) - 1; k >= 0 && rc == MPM_OK; --k) {
+            float* tk = s->tape + (size_t)k * st_words;
+            if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, tk)) != MPM_OK) break;
+            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[L], count(L), nullptr, dt, 0, 1, gb,
+                                               s->stream));
+            int blocks = 0;
+            mpm::AdjArgs q = adj_args(s, dt, L, tk, blocks);
+            HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
+            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, g, s->gadj, s->list[L], count(L), dt, gb, s->stream));
+            HIP_TRY(s, mpm::launch_adj_p2g(q, blocks, s->stream));
+            HIP_TRY(s, mpm::launch_clear_tiles(g, s->list[L], count(L), gb, s->stream, s->gadj));
+        }
+    }
+    s->list_cur = L;
+    s->grid_dirty = false;
+    s->p2g_mode = mode;
+    if (rc != MPM_OK) return rc;
+    HIP_TRY(s, hipMemcpyAsync(s->status + mpm::ST_GATHERS, &gathers, 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    s->adj_valid = true;
+    return MPM_OK;
+}
+
+int adj_layout(int field, int* comp0, int* ncomp) {
+    switch (field) {
+        case MPM_FIELD_POSITION: *comp0 = mpm::AX; *ncomp = 3; return 0;
+        case MPM_FIELD_VELOCITY: *comp0 = mpm::AV; *ncomp = 3; return 0;
+        case MPM_FIELD_C: *comp0 = mpm::AC; *ncomp = 9; return 0;
+        case MPM_FIELD_F: *comp0 = mpm::AF; *ncomp = 9; return 0;
+        case MPM_FIELD_YOUNGS: *comp0 = mpm::AE; *ncomp = 1; return 0;
+        default: return -1;
+    }
+}
+
+int read_adjoint(mpm_sim* s, int field, float* out) {
+    int c0, nc;
+    if (adj_layout(field, &c0, &nc) != 0) return fail(s, MPM_ERR_INVALID, "no adjoint for this field");
+    HIP_TRY(s, mpm::launch_unpermute(s->adj, s->cap, c0, nc, s->slot_of_id, s->staging, s->n, s->stream));
+    HIP_TRY(s, hipMemcpyAsync(out, s->staging, (size_t)s->n * nc * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    return MPM_OK;
+}
+
+int read_mu_lambda(mpm_sim* s, double* out) {
+    uint32_t nch = 0;
+    HIP_TRY(s, hipMemcpy(&nch, s->status + mpm::ST_NCHUNKS, 4, hipMemcpyDeviceToHost));
+    nch = std::min(nch, s->max_schunks);
+    std::vector<double> h(2 * (size_t)nch);
+    if (nch) HIP_TRY(s, hipMemcpy(h.data(), s->gml, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double m = 0.0, l = 0.0;
+    for (uint32_t i = 0; i < nch; ++i) {
+        m += h[2 * i];
+        l += h[2 * i + 1];
+    }
+    out[0] = m;
+    out[1] = l;
     return MPM_OK;
 }
 
@@ -926,22 +1132,27 @@ int mpm_reset_profile(mpm_sim* sim) {
 
 int mpm_diff_forward(mpm_sim* sim, float dt, uint32_t steps, uint32_t checkpoint_every) {
     if (!sim) return MPM_ERR_INVALID;
-    (void)dt;
-    (void)steps;
-    (void)checkpoint_every;
-    return fail(sim, MPM_ERR_INVALID, "differentiable path not built in this library version");
+    return diff_forward(sim, dt, steps, checkpoint_every);
 }
 
 int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad_v_final, float* grad_x0,
                       float* grad_v0, float* grad_E, double* grad_mu_lambda) {
     if (!sim) return MPM_ERR_INVALID;
-    (void)grad_x_final;
-    (void)grad_v_final;
-    (void)grad_x0;
-    (void)grad_v0;
-    (void)grad_E;
-    (void)grad_mu_lambda;
-    return fail(sim, MPM_ERR_INVALID, "differentiable path not built in this library version");
+    int rc = diff_backward(sim, grad_x_final, grad_v_final);
+    if (rc != MPM_OK) return rc;
+    if (grad_x0 && (rc = read_adjoint(sim, MPM_FIELD_POSITION, grad_x0)) != MPM_OK) return rc;
+    if (grad_v0 && (rc = read_adjoint(sim, MPM_FIELD_VELOCITY, grad_v0)) != MPM_OK) return rc;
+    if (grad_E && (rc = read_adjoint(sim, MPM_FIELD_YOUNGS, grad_E)) != MPM_OK) return rc;
+    if (grad_mu_lambda && (rc = read_mu_lambda(sim, grad_mu_lambda)) != MPM_OK) return rc;
+    return MPM_OK;
+}
+
+int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!sim->adj_valid) return fail(sim, MPM_ERR_INVALID, "mpm_diff_backward must run first");
+    if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "output length must equal the particle count");
+    if (!out) return fail(sim, MPM_ERR_INVALID, "out must not be NULL");
+    return read_adjoint(sim, field, out);
 }
 
 }  // extern "C"

@@ -114,6 +114,7 @@ _SIGS = {
     "mpm_diff_forward": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]),
     "mpm_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
+    "mpm_diff_read_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
 }
 
 
@@ -375,4 +376,14 @@ class MpmSimulation:
         ml = np.zeros(2, np.float64)
         self._check(lib().mpm_diff_backward(self._h, _ptr(gx), _ptr(gv), _ptr(ox), _ptr(ov),
                                             _ptr(oe), _ptr(ml)))
-        return {"x": ox, "v": ov, "E": oe, "mu": float(ml[0]), "lambda": float(ml[1])}
+        nu = float(self.cfgs[0].poisson)
+        g_youngs = ml[0] / (2 * (1 + nu)) + ml[1] * nu / ((1 + nu) * (1 - 2 * nu))
+        return {"x": ox, "v": ov, "E": oe, "mu": float(ml[0]), "lambda": float(ml[1]),
+                "youngs": float(g_youngs)}
+
+    def read_adjoint(self, name: str):
+        """dJ/d(initial field) of the last diff_backward, original particle order."""
+        fid, width = FIELDS[name]
+        out = np.zeros((self.n, width) if width > 1 else self.n, np.float32)
+        self._check(lib().mpm_diff_read_adjoint(self._h, fid, _ptr(out), self.n))
+        return out
