@@ -114,7 +114,7 @@ def cpu_baseline(cfg, d, dt, substeps):
     return n * substeps / el, el
 
 
-def run_batched(steps, warmup, scale):
+def run_batched(steps, warmup, scale, autotune=True):
     from molla_amd import MpmSimulation, workloads
 
     n_scenes = max(8, int(512 * scale))
@@ -122,6 +122,7 @@ def run_batched(steps, warmup, scale):
     sim = MpmSimulation(cfgs)
     sim.set_particles_batch(xs)
     dt = meta["dt"]
+    tuned = sim.autotune(dt, 10) if autotune else None
     if warmup:
         sim.step_n_graph(dt, steps)
     el = time_steps(sim, dt, steps, graph=True)
@@ -129,7 +130,70 @@ def run_batched(steps, warmup, scale):
     sim.close()
     return {"value": n * steps / el, "unit": "particle-substeps/s", "scenes": n_scenes,
             "particles": n, "substeps": steps, "ms_per_step": 1e3 * el / steps,
-            "grids": [meta["grid_min"], meta["grid_max"]], "graph": True}
+            "grids": [meta["grid_min"], meta["grid_max"]], "graph": True, "autotuned": tuned}
+
+
+def run_plain(scale):
+    """The 8,192-particle elastic cube on a 32^3 grid, one scene, 1000
+    substeps in one step_n call (the reference's plain single-simulation
+    configuration)."""
+    from molla_amd import MpmSimulation, workloads
+
+    n = max(1024, int(8192 * min(scale, 1.0)))
+    cfg, x, meta = workloads.elastic_cube(n, seed=1)
+    sim = MpmSimulation(cfg, max_particles=n)
+    sim.set_particles(x, None)
+    sim.step_n(meta["dt"], 20)
+    el = time_steps(sim, meta["dt"], 1000, graph=False)
+    sim.close()
+    return {"workload": "elastic_cube", "particles": n, "grid": cfg.grid_size, "substeps": 1000,
+            "value": n * 1000 / el, "unit": "particle-substeps/s", "ms_per_step": 1e3 * el / 1000}
+
+
+def run_adjoint(scale):
+    """Forward with checkpoints + reverse pass of the 65,536-particle diff scene
+    (64 substeps, checkpoint every 16, COM loss); reports the reverse/forward
+    time ratio."""
+    from molla_amd import MpmSimulation, workloads
+
+    n = max(4096, int(65536 * scale))
+    cfg, d, meta = workloads.diff_scene(n)
+    steps, every, dt = meta["substeps"], meta["checkpoint_every"], meta["dt"]
+    target = np.asarray(meta["target"])
+    sim = MpmSimulation(cfg, max_particles=n)
+
+    def reset():
+        sim.set_particles(d["x"], d["v"])
+        sim.write_field("youngs", d["E"])
+
+    def loss_grad():
+        xT = sim.read_positions().astype(np.float64)
+        r = xT.mean(axis=0) - target
+        return float(r @ r), np.tile((2.0 / n) * r, (n, 1)).astype(np.float32)
+
+    reset()
+    sim.diff_forward(dt, steps, every)
+    sim.diff_backward(loss_grad()[1])
+    reset()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.diff_forward(dt, steps, every)
+    t_fwd = time.perf_counter() - t0
+    J, gx = loss_grad()
+    t0 = time.perf_counter()
+    g = sim.diff_backward(gx)
+    t_bwd = time.perf_counter() - t0
+    reset()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step_n(dt, steps)
+    sim.synchronize()
+    t_plain = time.perf_counter() - t0
+    sim.close()
+    return {"workload": "diff_scene", "particles": n, "grid": meta["grid"], "substeps": steps,
+            "checkpoint_every": every, "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
+            "adjoint_fwd_ratio": t_bwd / t_fwd, "plain_step_n_ms": 1e3 * t_plain,
+            "loss": J, "dJ_dyoungs_sum": float(np.sum(g["E"], dtype=np.float64))}
 
 
 def main():
@@ -141,6 +205,7 @@ def main():
     ap.add_argument("--cpu-substeps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay step_n as a hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB (for counter scaling)")
@@ -199,6 +264,21 @@ def main():
             log(f"batched: {out['batched']}")
         except Exception as e:  # reported, never silently replaced
             out["batched"] = {"error": str(e)}
+
+    try:
+        out["plain"] = run_plain(args.scale)
+        log(f"plain: {out['plain']}")
+        if "batched" in out and "value" in out["batched"]:
+            out["batched"]["vs_plain"] = out["batched"]["value"] / out["plain"]["value"]
+    except Exception as e:  # reported, never silently replaced
+        out["plain"] = {"error": str(e)}
+
+    if not args.no_adjoint:
+        try:
+            out["adjoint"] = run_adjoint(args.scale)
+            log(f"adjoint: {out['adjoint']}")
+        except Exception as e:  # reported, never silently replaced
+            out["adjoint"] = {"error": str(e)}
 
     if not args.no_cpu:
         sim.close()

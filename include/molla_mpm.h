@@ -156,6 +156,16 @@ int mpm_set_write_vc(mpm_sim* sim, int every_substep);
 /* Launch tuning: blocks for the particle pass (0 = one per chunk). */
 int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks);
 
+/* Launch autotuner.  No counterpart in the reference, whose dispatch sizes
+ * are fixed; it serves the batched mode, where scene mixes differ widely.
+ * Times `trial_substeps` substeps (after a 2-substep warm-up) for candidate
+ * persistent particle-pass grids, grid-update grids and sort intervals
+ * (coordinate search), applies the fastest as mpm_set_launch /
+ * mpm_set_sort_interval would, and restores the particle state it started
+ * from (stepping resumes from the same state).  Outputs may be NULL. */
+int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,
+                 uint32_t* grid_blocks, uint32_t* sort_every);
+
 /* ---- stepping (step / step_n, mpm.rs:2232-2272) ---- */
 int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps);
 /* Capture step_n(dt, substeps) into a hipGraph once and replay it

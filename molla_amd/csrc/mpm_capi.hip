@@ -113,6 +113,7 @@ struct mpm_sim {
     uint32_t diff_sorts = 0;
     int diff_ps = 0;
     float* tape = nullptr;     // pre-states of one checkpoint segment (NCOMP planes each)
+    float4* gtape = nullptr;   // per substep of the segment: grid v, then P2G total (m, p)
     uint32_t tape_len = 0;
     float* adj = nullptr;      // NADJ adjoint planes
     float4* gadj = nullptr;    // per-cell grid adjoint
@@ -204,7 +205,7 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->adj, s->gadj, s->gml};
+    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml};
     for (void* b : dbufs)
         if (b) hipFree(b);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -728,8 +729,12 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     const size_t cells = (size_t)s->total_tiles * 64;
     if (s->tape_len < every) {
         if (s->tape) HIP_TRY(s, hipFree(s->tape));
+        if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
         s->tape = nullptr;
+        s->gtape = nullptr;
+        s->tape_len = 0;
         HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * plane));
+        HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
         s->tape_len = every;
     }
     if (!s->adj) HIP_TRY(s, dalloc(&s->adj, (size_t)mpm::NADJ * plane));
@@ -768,30 +773,47 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
         HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
         HIP_TRY(s, hipMemcpyAsync(t0 + 24 * plane, s->ps[s->ps_cur] + 24 * plane, (size_t)(mpm::NCOMP - 24) * plane * 4,
                                   hipMemcpyDeviceToDevice, s->stream));
-        // recompute the pre-states of substeps 1 .. len-1 of the segment
-        for (uint32_t k = 0; k + 1 < len && rc == MPM_OK; ++k) {
+        // Recompute the segment with the fused forward: substep k's grid goes
+        // to the grid tape and its pre-state to tape slot k.
+        if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0)) != MPM_OK) break;
+        for (uint32_t k = 0; k < len && rc == MPM_OK; ++k) {
             float* tk = s->tape + k * st_words;
+            mpm::GridDev gk = g;
+            gk.gv = s->gtape + (size_t)2 * k * cells;
+            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
+                                               s->stream, gk.gv + cells));
+            if (k + 1 == len) break;
             float* tk1 = tk + st_words;
             HIP_TRY(s, hipMemcpyAsync(tk1, tk, st_words * 4, hipMemcpyDeviceToDevice, s->stream));
-            if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, tk1)) != MPM_OK) break;
-            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
-                                               s->stream));
-            if ((rc = particle_pass(s, mpm::MODE_G2P, dt, L, tk1)) != MPM_OK) break;
+            mpm::StepArgs a = args(s, dt, 1 - L);
+            a.ps = tk1;
+            a.gv = gk.gv;
+            a.write_vc = 1;
+            int blocks = 0;
+            use_sorted_chunks(s, a, blocks);
+            HIP_TRY(s, mpm::launch_particle_pass(mpm::MODE_G2P2G, int(s->sh[0].model), a, 1, blocks, s->stream));
             L = 1 - L;
         }
-        // reverse the segment's substeps
+        if (rc != MPM_OK) break;
+        // The last grid update reset the other list; the reverse builds its
+        // own list of escaped tiles there.
+        const int R = 1 - L;
         for (int k = int(len) - 1; k >= 0 && rc == MPM_OK; --k) {
             float* tk = s->tape + (size_t)k * st_words;
-            if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, tk)) != MPM_OK) break;
-            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[L], count(L), nullptr, dt, 0, 1, gb,
-                                               s->stream));
+            float4* gvk = s->gtape + (size_t)2 * k * cells;
             int blocks = 0;
-            mpm::AdjArgs q = adj_args(s, dt, L, tk, blocks);
+            mpm::AdjArgs q = adj_args(s, dt, R, tk, blocks);
+            q.a.gv = gvk;
             HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
-            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, g, s->gadj, s->list[L], count(L), dt, gb, s->stream));
+            mpm::GridDev gk = g;
+            gk.gmp = gvk + cells;
+            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, gb, s->stream));
             HIP_TRY(s, mpm::launch_adj_p2g(q, blocks, s->stream));
-            HIP_TRY(s, mpm::launch_clear_tiles(g, s->list[L], count(L), gb, s->stream, s->gadj));
+            mpm::GridDev gc = g;
+            gc.gmp = s->gadj;
+            HIP_TRY(s, mpm::launch_clear_tiles(gc, s->list[R], count(R), gb, s->stream));
         }
+        L = R;
     }
     s->list_cur = L;
     s->grid_dirty = false;
@@ -836,6 +858,103 @@ int read_mu_lambda(mpm_sim* s, double* out) {
     }
     out[0] = m;
     out[1] = l;
+    return MPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Launch autotuner: coordinate search over the persistent particle-pass grid,
+// the grid-update grid and the sort interval, timed with HIP events on the
+// context's stream.  The particle state is backed up first and restored
+// afterwards (unsorted, so the next step re-sorts).
+int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* out_gb, uint32_t* out_se) {
+    if (!std::isfinite(dt) || dt < 0.0f) return fail(s, MPM_ERR_INVALID, "dt must be finite and >= 0");
+    if (trial == 0) return fail(s, MPM_ERR_INVALID, "trial_substeps must be >= 1");
+    if (s->n == 0) return MPM_OK;
+    int rc = ensure_grid(s);
+    if (rc != MPM_OK) return rc;
+    const size_t words = (size_t)mpm::NCOMP * s->cap;
+    float* backup = nullptr;
+    HIP_TRY(s, dalloc(&backup, words));
+    HIP_TRY(s, hipMemcpyAsync(backup, s->ps[s->ps_cur], words * 4, hipMemcpyDeviceToDevice, s->stream));
+    uint32_t st[mpm::ST_WORDS];
+    HIP_TRY(s, hipMemcpyAsync(st, s->status, sizeof(st), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    const uint32_t sorts0 = s->sorts;
+    const bool prof = s->profiling;
+    s->profiling = false;
+    hipEvent_t ea = nullptr, eb = nullptr;
+    auto restore = [&]() -> int {
+        int r = clear_grid(s);
+        if (r != MPM_OK) return r;
+        HIP_TRY(s, hipMemcpyAsync(s->ps[0], backup, words * 4, hipMemcpyDeviceToDevice, s->stream));
+        s->ps_cur = 0;
+        HIP_TRY(s, mpm::launch_index_slots(s->ps[0], s->cap, s->slot_of_id, s->n, s->stream));
+        s->sorted = false;
+        return MPM_OK;
+    };
+    auto run_trial = [&](uint32_t pb, uint32_t gb, uint32_t se, float& ms) -> int {
+        int r = restore();
+        if (r != MPM_OK) return r;
+        s->pblocks = pb;
+        s->gblocks = gb;
+        s->sort_every = se;
+        if ((r = run_step_n(s, dt, 2)) != MPM_OK) return r;
+        HIP_TRY(s, hipEventRecord(ea, s->stream));
+        if ((r = run_step_n(s, dt, trial)) != MPM_OK) return r;
+        HIP_TRY(s, hipEventRecord(eb, s->stream));
+        HIP_TRY(s, hipEventSynchronize(eb));
+        HIP_TRY(s, hipEventElapsedTime(&ms, ea, eb));
+        return MPM_OK;
+    };
+    uint32_t best_pb = s->pblocks, best_gb = s->gblocks, best_se = s->sort_every;
+    if (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess) rc = fail(s, MPM_ERR_DEVICE, "event");
+    float best = 1e30f, ms = 0.f;
+    const uint32_t pbs[] = {0u, 1024u, 2048u, 4096u, 8192u, 16384u};
+    const uint32_t gbs[] = {512u, 1024u, 2048u, 4096u, 8192u};
+    const uint32_t ses[] = {8u, 16u, 32u};
+    const uint32_t gb0 = best_gb, se0 = best_se ? best_se : 16u;
+    for (uint32_t pb : pbs) {
+        if (rc != MPM_OK) break;
+        if ((rc = run_trial(pb, gb0, se0, ms)) == MPM_OK && ms < best) {
+            best = ms;
+            best_pb = pb;
+        }
+    }
+    best = 1e30f;
+    for (uint32_t gb : gbs) {
+        if (rc != MPM_OK) break;
+        if ((rc = run_trial(best_pb, gb, se0, ms)) == MPM_OK && ms < best) {
+            best = ms;
+            best_gb = gb;
+        }
+    }
+    best = 1e30f;
+    for (uint32_t se : ses) {
+        if (rc != MPM_OK) break;
+        if ((rc = run_trial(best_pb, best_gb, se, ms)) == MPM_OK && ms < best) {
+            best = ms;
+            best_se = se;
+        }
+    }
+    int rr = restore();
+    if (rc == MPM_OK) rc = rr;
+    if (ea) hipEventDestroy(ea);
+    if (eb) hipEventDestroy(eb);
+    s->profiling = prof;
+    s->pblocks = best_pb;
+    s->gblocks = best_gb;
+    s->sort_every = best_se;
+    s->sorts = sorts0;
+    // counters of the trials are not the caller's: put back the fault latch,
+    // gathers, escapes and fallback counters
+    const int keep[] = {mpm::ST_FAULT, mpm::ST_GATHERS, mpm::ST_ESCAPES, mpm::ST_FALLBACK};
+    for (int w : keep) HIP_TRY(s, hipMemcpyAsync(s->status + w, &st[w], 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    HIP_TRY(s, hipFree(backup));
+    if (rc != MPM_OK) return rc;
+    if (out_pb) *out_pb = best_pb;
+    if (out_gb) *out_gb = best_gb;
+    if (out_se) *out_se = best_se;
     return MPM_OK;
 }
 
@@ -1145,6 +1264,12 @@ int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad
     if (grad_E && (rc = read_adjoint(sim, MPM_FIELD_YOUNGS, grad_E)) != MPM_OK) return rc;
     if (grad_mu_lambda && (rc = read_mu_lambda(sim, grad_mu_lambda)) != MPM_OK) return rc;
     return MPM_OK;
+}
+
+int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,
+                 uint32_t* grid_blocks_, uint32_t* sort_every) {
+    if (!sim) return MPM_ERR_INVALID;
+    return autotune(sim, dt, trial_substeps, particle_blocks, grid_blocks_, sort_every);
 }
 
 int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n) {

@@ -132,9 +132,11 @@ struct StepArgs {
 // ---- launchers (mpm_kernels.hip) ----
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
                                 hipStream_t s);
+// keep (may be null): per-cell copy of the P2G total (m, p) of the listed tiles.
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
-                              int zero_after, int use_partials, int blocks, hipStream_t s);
+                              int zero_after, int use_partials, int blocks, hipStream_t s,
+                              float4* keep = nullptr);
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also = nullptr);
 // Static active list of a sort: the 3x3x3 dilation of occupied tiles.
@@ -168,7 +170,8 @@ hipError_t launch_reorder_index(const uint32_t* sorted_ids, const uint32_t* slot
 enum AdjComp : int { AX = 0, AV = 3, AC = 6, AF = 15, AE = 24, NADJ = 25 };
 
 struct AdjArgs {
-    StepArgs a;    // a.ps: the substep's pre-state; a.gmp: atomic target of the reverse G2P scatter
+    StepArgs a;    // a.ps: the substep's pre-state; a.gv: its grid velocity;
+                   // a.gmp: atomic target of the reverse G2P scatter
     float* adj;    // NADJ planes of a.cap floats
     float4* gadj;  // per cell: (dJ/dm, dJ/dp) after the grid reverse
     double* gml;   // per chunk: accumulated (dJ/dmu, dJ/dlambda)
@@ -176,7 +179,8 @@ struct AdjArgs {
 // Reverse G2P + x/F advance: gathers grid v, scatters dJ/d(grid v) (partials),
 // folds the G2P part of dJ/dx into adj x and replaces adj F by (I + dt C)^T adj F.
 hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s);
-// Grid reverse over the active tiles: dJ/d(grid v) -> (dJ/dm, dJ/dp) in gadj.
+// Grid reverse over the active tiles: dJ/d(grid v) -> (dJ/dm, dJ/dp) in gadj;
+// g.gmp holds the substep's P2G total (m, p).
 hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
                            const uint32_t* dcount, float dt, int blocks, hipStream_t s);
 // Reverse P2G + stress: gathers gadj; writes adj x, v, C, F of the pre-state

@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
-                                                     int zero_after, int use_partials) {
+                                                     int zero_after, int use_partials, float4* keep) {
     const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
         float4 out;
         cell_update(sc, q, ix, iy, iz, dt, out);
         g.gv[c] = out;
+        if (keep) keep[c] = q;  // the P2G total, for the reverse pass
         if (zero_after) {
             if (flag & FLAG_GMP) g.gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (lane == 0) g.tile_flag[t] = flag & FLAG_STATIC;
@@ -440,9 +441,9 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
 
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
-                              int zero_after, int use_partials, int blocks, hipStream_t s) {
+                              int zero_after, int use_partials, int blocks, hipStream_t s, float4* keep) {
     hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count, dt,
-                       zero_after, use_partials);
+                       zero_after, use_partials, keep);
     return hipGetLastError();
 }
 

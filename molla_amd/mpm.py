@@ -115,6 +115,7 @@ _SIGS = {
     "mpm_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "mpm_diff_read_adjoint": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    "mpm_autotune": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 
@@ -360,6 +361,14 @@ class MpmSimulation:
 
     def reset_profile(self):
         self._check(lib().mpm_reset_profile(self._h))
+
+    def autotune(self, dt: float, trial_substeps: int = 10) -> dict:
+        """Pick the fastest launch sizes and sort interval for this scene mix
+        (the particle state is restored afterwards)."""
+        pb, gb, se = (C.c_uint32(), C.c_uint32(), C.c_uint32())
+        self._check(lib().mpm_autotune(self._h, float(dt), int(trial_substeps), C.byref(pb), C.byref(gb),
+                                       C.byref(se)))
+        return {"particle_blocks": pb.value, "grid_blocks": gb.value, "sort_every": se.value}
 
     # -- differentiable path --
     def diff_forward(self, dt: float, steps: int, checkpoint_every: int = 16):

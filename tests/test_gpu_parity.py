@@ -283,3 +283,22 @@ def test_escaped_particles_fall_back_to_global_atomics():
     e, e32 = rel_err(sim.read_positions(), o64.x), rel_err(o32.x, o64.x)
     log(f"  x err {e:.2e} (f32 oracle {e32:.2e})")
     assert e <= max(8 * e32, 1e-6)
+
+
+def test_autotune_restores_state_and_keeps_results():
+    cfg, x, meta = workloads.elastic_cube(8192, seed=5)
+    dt = meta["dt"]
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x, None)
+    x0, F0 = sim.read_positions(), sim.read_field("F")
+    tuned = sim.autotune(dt, 4)
+    log(f"autotune -> {tuned}")
+    assert np.array_equal(sim.read_positions(), x0) and np.array_equal(sim.read_field("F"), F0)
+    st = sim.read_status()
+    assert st["fault_bits"] == 0 and st["completed_gathers"] == 0 and st["sorts"] == 0
+    sim.step_n(dt, 30)
+    _, ref = run_gpu(cfg, x, None, dt, 30)
+    e = rel_err(sim.read_positions(), ref["x"])
+    log(f"  tuned vs default x: {e:.2e}")
+    assert e < 1e-6
+    assert sim.read_status()["completed_gathers"] == 30
