@@ -2,7 +2,7 @@
 //
 // * quadratic B-spline weights (mpm.wgsl:287-294)
 // * FP64 3x3 SVD by cyclic Jacobi on F^T F: the rotation angle is estimated
-//   with the fast f32 rsqrt/rcp units and the rotation is renormalised and
+//   with the hardware f32 rcp/sqrt/rsq and the rotation is renormalised and
 //   applied in f64, so the factorisation is accurate to f64 rounding while
 //   avoiding f64 division and square root (each slow on CDNA).  sigma_i and
 //   1/sigma_i come from an f32 rsqrt seed refined by two f64 Newton steps.
@@ -52,10 +52,12 @@ struct JacobiPair {
     __device__ static __forceinline__ void rot(double& app, double& aqq, double& apq, double& akp,
                                                double& akq, double* vp, double* vq, double tol) {
         if (fabs(apq) <= tol) return;
-        float th = (float)(aqq - app) / (2.0f * (float)apq);
-        float t = 1.0f / (fabsf(th) + sqrtf(1.0f + th * th));
+        // Angle from the hardware rcp/sqrt/rsq estimates: any rotation near
+        // the Jacobi angle converges, and (c, s) is renormalised in f64 below.
+        float th = (float)(aqq - app) * __builtin_amdgcn_rcpf(2.0f * (float)apq);
+        float t = __builtin_amdgcn_rcpf(fabsf(th) + __builtin_amdgcn_sqrtf(1.0f + th * th));
         t = th >= 0.0f ? t : -t;
-        float cf = __frsqrt_rn(1.0f + t * t);
+        float cf = __builtin_amdgcn_rsqf(1.0f + t * t);
         double c = cf, s = (double)(t * cf);
         double r = c * c + s * s;
         double k = 1.5 - 0.5 * r;
@@ -83,16 +85,16 @@ struct JacobiPair {
 
 // F (row-major f32) -> U, sigma (>= 0), V in f64 with V proper.
 __device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* sg, double* V) {
-    double f[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) f[i] = (double)F[i];
-    // A = F^T F; column j of F is (f[j], f[3+j], f[6+j]).
-    double a00 = f[0] * f[0] + f[3] * f[3] + f[6] * f[6];
-    double a11 = f[1] * f[1] + f[4] * f[4] + f[7] * f[7];
-    double a22 = f[2] * f[2] + f[5] * f[5] + f[8] * f[8];
-    double a01 = f[0] * f[1] + f[3] * f[4] + f[6] * f[7];
-    double a02 = f[0] * f[2] + f[3] * f[5] + f[6] * f[8];
-    double a12 = f[1] * f[2] + f[4] * f[5] + f[7] * f[8];
+    // F stays f32 (exact in f64) and is widened where used, so no f64 copy
+    // of it is live across the sweeps.
+#define MPM_F(i) ((double)F[i])
+    // A = F^T F; column j of F is (F[j], F[3+j], F[6+j]).
+    double a00 = MPM_F(0) * MPM_F(0) + MPM_F(3) * MPM_F(3) + MPM_F(6) * MPM_F(6);
+    double a11 = MPM_F(1) * MPM_F(1) + MPM_F(4) * MPM_F(4) + MPM_F(7) * MPM_F(7);
+    double a22 = MPM_F(2) * MPM_F(2) + MPM_F(5) * MPM_F(5) + MPM_F(8) * MPM_F(8);
+    double a01 = MPM_F(0) * MPM_F(1) + MPM_F(3) * MPM_F(4) + MPM_F(6) * MPM_F(7);
+    double a02 = MPM_F(0) * MPM_F(2) + MPM_F(3) * MPM_F(5) + MPM_F(6) * MPM_F(8);
+    double a12 = MPM_F(1) * MPM_F(2) + MPM_F(4) * MPM_F(5) + MPM_F(7) * MPM_F(8);
     // V stored by columns: v0, v1, v2.
     double v0[3] = {1.0, 0.0, 0.0}, v1[3] = {0.0, 1.0, 0.0}, v2[3] = {0.0, 0.0, 1.0};
     const double tol = 1.0e-15 * (a00 + a11 + a22);
@@ -106,9 +108,9 @@ __device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* 
     double* vc[3] = {v0, v1, v2};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        double w0 = f[0] * vc[i][0] + f[1] * vc[i][1] + f[2] * vc[i][2];
-        double w1 = f[3] * vc[i][0] + f[4] * vc[i][1] + f[5] * vc[i][2];
-        double w2 = f[6] * vc[i][0] + f[7] * vc[i][1] + f[8] * vc[i][2];
+        double w0 = MPM_F(0) * vc[i][0] + MPM_F(1) * vc[i][1] + MPM_F(2) * vc[i][2];
+        double w1 = MPM_F(3) * vc[i][0] + MPM_F(4) * vc[i][1] + MPM_F(5) * vc[i][2];
+        double w2 = MPM_F(6) * vc[i][0] + MPM_F(7) * vc[i][1] + MPM_F(8) * vc[i][2];
         double n2 = w0 * w0 + w1 * w1 + w2 * w2;
         if (n2 > 1.0e-24) {
             double r = rsqrt_f64(n2);
@@ -126,6 +128,7 @@ __device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* 
         V[1 * 3 + i] = vc[i][1];
         V[2 * 3 + i] = vc[i][2];
     }
+#undef MPM_F
 }
 
 __device__ __forceinline__ double det3d(const double* m) {

@@ -64,7 +64,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_g2p(AdjArgs q) {
     float* __restrict__ adj = q.adj;
     const float dt = a.dt;
     const uint32_t nch = *a.n_chunks_dev;
-    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+    for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
+        const uint32_t ci = w.ci;
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
@@ -221,7 +222,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_p2g(AdjArgs q) {
     const float dt = a.dt;
     const uint32_t nch = *a.n_chunks_dev;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+    for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
+        const uint32_t ci = w.ci;
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
@@ -435,6 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_p2g(AdjArgs q) {
 
 hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
+    if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);
     hipLaunchKernelGGL(k_adj_g2p, dim3(blocks), dim3(BLOCK), 0, s, q);
     return hipGetLastError();
 }
@@ -447,6 +450,7 @@ hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gad
 
 hipError_t launch_adj_p2g(const AdjArgs& q, int blocks, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
+    if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);
     hipLaunchKernelGGL(k_adj_p2g, dim3(blocks), dim3(BLOCK), 0, s, q);
     return hipGetLastError();
 }

@@ -89,7 +89,11 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 // tile lower-bound table of the last sort and sums their partials per cell.
 // A particle whose stencil has left that neighbourhood since the sort is
 // scattered with global atomics instead (and counted in ST_ESCAPES).
-constexpr int LDS_CELLS = 2048;
+// 1600 cells of four int64 channels (51 KB) lets three blocks share a CU's
+// 160 KB LDS; a chunk's box (four tiles of one tile row plus the stencil
+// reach and the motion since the last sort) stays well inside it, and a
+// larger box falls back to global atomics (counted in ST_FALLBACK).
+constexpr int LDS_CELLS = 1600;
 
 struct GridDev {
     float4* gmp;          // atomic accumulation / final P2G kept for readback

@@ -57,8 +57,10 @@ __device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float*
     for (int i = 0; i < 9; ++i) A[i] = m * C[i] + k * S[i];
 }
 
+// Three waves per SIMD (168 VGPRs) with three blocks per CU (LDS_CELLS):
+// the pass is latency-bound, and this beats the unconstrained 2 waves/SIMD.
 template <int MODE, int MODEL>
-__global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_particles(StepArgs a) {
     __shared__ ScatterLds L;
     if (threadIdx.x == 0) L.box[15] = a.status[ST_FAULT] != 0u ? 1 : 0;
     __syncthreads();
@@ -68,7 +70,8 @@ __global__ __launch_bounds__(BLOCK) void k_particles(StepArgs a) {
     float* __restrict__ ps = a.ps;
     const float dt = a.dt;
     const uint32_t nch = a.n_chunks_dev ? *a.n_chunks_dev : a.n_chunks;
-    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+    for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
+        const uint32_t ci = w.ci;
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
@@ -349,6 +352,7 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
     (void)ppt;
     if (a.n_chunks == 0) return hipSuccess;
     if (blocks <= 0 || uint32_t(blocks) > a.n_chunks) blocks = int(a.n_chunks);
+    if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);  // XCD-aware chunk walk
     blocks = std::max(blocks, 1);
     dim3 g(blocks), b(BLOCK);
 #define MPM_LAUNCH(M, D) hipLaunchKernelGGL((k_particles<M, D>), g, b, 0, s, a)

@@ -11,6 +11,24 @@ namespace mpm {
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 
+// XCD-aware walk of a chunk list by a persistent grid.  Workgroups are
+// dispatched round-robin over the 8 XCDs of an MI355X, each with its own L2,
+// so block b runs on XCD b % 8.  Giving every XCD one contiguous range of the
+// (spatially sorted) chunk list keeps the grid tiles its chunks touch in that
+// XCD's L2 instead of replicating the whole grid in all eight.
+constexpr uint32_t NUM_XCD = 8;
+struct ChunkWalk {
+    uint32_t ci, end, step;
+};
+__device__ __forceinline__ ChunkWalk chunk_walk(uint32_t nch) {
+    const uint32_t g = gridDim.x;
+    if (g < NUM_XCD || g % NUM_XCD != 0) return {blockIdx.x, nch, g};
+    const uint32_t x = blockIdx.x % NUM_XCD;
+    const uint32_t lo = uint32_t((uint64_t)nch * x / NUM_XCD);
+    const uint32_t hi = uint32_t((uint64_t)nch * (x + 1) / NUM_XCD);
+    return {lo + blockIdx.x / NUM_XCD, hi, g / NUM_XCD};
+}
+
 __device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int iy, int iz) {
     uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
     return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
