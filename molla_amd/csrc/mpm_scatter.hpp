@@ -294,36 +294,97 @@ __device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDe
 // (ix, iy, iz) of tile (tx, ty, tz): the chunks holding particles sorted into
 // the tile's 3x3x3 neighbourhood, i.e. nine x-rows of consecutive tile keys
 // whose slot ranges come from the tile lower-bound table.  A chunk spanning
-// two rows is added once (ascending chunk ids), so the sum order is fixed.
+// two rows is added once, and chunks are added in ascending id, so the sum
+// order is fixed.  Called by all 64 lanes of a wave for one tile:
+//   1. lanes 0-8 resolve the nine rows' chunk ranges at once,
+//   2. the deduplicated ranges are laid out as one candidate list, one
+//      candidate per lane, whose boxes load in parallel,
+//   3. a ballot keeps the boxes overlapping the tile, and the wave adds
+//      their partials in ascending order, two loads in flight at a time.
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
                                              int ix, int iy, int iz, float4& q) {
+    const int lane = threadIdx.x & 63;
     const int W = int(sc.W);
-    int last = -1;
+    const int X0 = tx * 4, Y0 = ty * 4, Z0 = tz * 4;
     const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int nz = tz + dz;
-        if (nz < 0 || nz >= W) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int ny = ty + dy;
-            if (ny < 0 || ny >= W) continue;
+    int rc0 = 0, rc1 = -1;  // empty row
+    if (lane < 9) {
+        const int nz = tz + lane / 3 - 1, ny = ty + lane % 3 - 1;
+        if (nz >= 0 && nz < W && ny >= 0 && ny < W) {
             const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));
             const uint32_t s0 = g.tile_lb[row + xa], s1 = g.tile_lb[row + xb + 1];
-            if (s0 >= s1) continue;
-            const int c0 = max(int(g.chunk_of_slot[s0]), last + 1);
-            const int c1 = min(int(g.chunk_of_slot[s1 - 1]), int(g.part_capacity) - 1);
-            for (int chn = c0; chn <= c1; ++chn) {
-                const int4 o = g.chunk_box[2 * (size_t)chn];
-                const int4 d = g.chunk_box[2 * (size_t)chn + 1];
-                const int lx = ix - o.x, ly = iy - o.y, lz = iz - o.z;
-                if (lx >= 0 && ly >= 0 && lz >= 0 && lx < d.x && ly < d.y && lz < d.z) {
-                    const float4 r = g.part[(size_t)chn * LDS_CELLS + lx + d.x * (ly + d.y * lz)];
-                    q.x += r.x;
-                    q.y += r.y;
-                    q.z += r.z;
-                    q.w += r.w;
-                }
+            if (s0 < s1) {
+                rc0 = int(g.chunk_of_slot[s0]);
+                rc1 = min(int(g.chunk_of_slot[s1 - 1]), int(g.part_capacity) - 1);
             }
-            last = max(last, c1);
+        }
+    }
+    int c0[9], off[10];
+    int last = -1, total = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int a0 = __builtin_amdgcn_readlane(rc0, r), a1 = __builtin_amdgcn_readlane(rc1, r);
+        const int lo = max(a0, last + 1);
+        c0[r] = lo;
+        off[r] = total;
+        total += a1 >= lo ? a1 - lo + 1 : 0;
+        last = max(last, a1);
+    }
+    off[9] = total;
+    for (int base = 0; base < total; base += 64) {
+        const int j = base + lane;
+        int mine = -1;
+        if (j < total) {
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+                if (j >= off[r] && j < off[r + 1]) mine = c0[r] + (j - off[r]);
+        }
+        int4 o = make_int4(0, 0, 0, 0), d = make_int4(0, 0, 0, 0);
+        bool hit = false;
+        if (mine >= 0) {
+            o = g.chunk_box[2 * (size_t)mine];
+            d = g.chunk_box[2 * (size_t)mine + 1];
+            hit = d.x > 0 && o.x < X0 + 4 && o.x + d.x > X0 && o.y < Y0 + 4 && o.y + d.y > Y0 && o.z < Z0 + 4 &&
+                  o.z + d.z > Z0;
+        }
+        uint64_t m = __ballot(hit);
+        while (m) {
+            const int l0 = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            const bool two = m != 0;
+            const int l1 = two ? __ffsll((unsigned long long)m) - 1 : l0;
+            if (two) m &= m - 1;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
+            {
+                const int ox = __builtin_amdgcn_readlane(o.x, l0), oy = __builtin_amdgcn_readlane(o.y, l0),
+                          oz = __builtin_amdgcn_readlane(o.z, l0);
+                const int bx = __builtin_amdgcn_readlane(d.x, l0), by = __builtin_amdgcn_readlane(d.y, l0),
+                          bz = __builtin_amdgcn_readlane(d.z, l0);
+                const int cid = __builtin_amdgcn_readlane(mine, l0);
+                const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
+                if (lx >= 0 && ly >= 0 && lz >= 0 && lx < bx && ly < by && lz < bz)
+                    r0 = g.part[(size_t)cid * LDS_CELLS + lx + bx * (ly + by * lz)];
+            }
+            if (two) {
+                const int ox = __builtin_amdgcn_readlane(o.x, l1), oy = __builtin_amdgcn_readlane(o.y, l1),
+                          oz = __builtin_amdgcn_readlane(o.z, l1);
+                const int bx = __builtin_amdgcn_readlane(d.x, l1), by = __builtin_amdgcn_readlane(d.y, l1),
+                          bz = __builtin_amdgcn_readlane(d.z, l1);
+                const int cid = __builtin_amdgcn_readlane(mine, l1);
+                const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
+                if (lx >= 0 && ly >= 0 && lz >= 0 && lx < bx && ly < by && lz < bz)
+                    r1 = g.part[(size_t)cid * LDS_CELLS + lx + bx * (ly + by * lz)];
+            }
+            q.x += r0.x;
+            q.y += r0.y;
+            q.z += r0.z;
+            q.w += r0.w;
+            if (two) {
+                q.x += r1.x;
+                q.y += r1.y;
+                q.z += r1.z;
+                q.w += r1.w;
+            }
         }
     }
 }
