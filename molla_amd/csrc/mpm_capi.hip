@@ -74,6 +74,9 @@ struct mpm_sim {
     uint32_t* cflags = nullptr;
     uint32_t* cexcl = nullptr;
     uint32_t* scan_scratch = nullptr;
+    uint32_t* smark = nullptr;  // static-list rebuild: per-tile mark, its scan, scan scratch
+    uint32_t* spos = nullptr;
+    uint32_t* sscan = nullptr;
     uint32_t* list[2] = {nullptr, nullptr};
     uint32_t* status = nullptr;  // ST_WORDS u32
     uint32_t* slot_of_id = nullptr;
@@ -201,7 +204,7 @@ void release(mpm_sim* s) {
                     s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
                     s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
                     s->src_slot, s->chunk_box, s->part, s->schunks, s->chunk_of_slot, s->cflags,
-                    s->cexcl, s->scan_scratch};
+                    s->cexcl, s->scan_scratch, s->smark, s->spos, s->sscan};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
@@ -316,6 +319,9 @@ int ensure_grid(mpm_sim* s) {
     HIP_TRY(s, dalloc(&s->cflags, s->cap));
     HIP_TRY(s, dalloc(&s->cexcl, s->cap));
     HIP_TRY(s, dalloc(&s->scan_scratch, mpm::scan_scratch_words(s->cap)));
+    HIP_TRY(s, dalloc(&s->smark, s->total_tiles));
+    HIP_TRY(s, dalloc(&s->spos, s->total_tiles));
+    HIP_TRY(s, dalloc(&s->sscan, mpm::scan_scratch_words(s->total_tiles)));
     HIP_TRY(s, hipMemsetAsync(s->chunk_box, 0, 2 * (size_t)s->part_cap * sizeof(int4), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gmp, 0, cells * sizeof(float4), s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gv, 0, cells * sizeof(float4), s->stream));
@@ -363,7 +369,11 @@ int sort_particles(mpm_sim* s, bool reorder) {
     }
     Stage st(s, MPM_STAGE_SORT);
     mpm::StepArgs a = args(s, 0.f, s->list_cur);
-    HIP_TRY(s, mpm::launch_tile_keys(a, s->keys, s->stream));
+    // The fused layout only needs particles grouped by tile: ties keep the
+    // current slot order.  The deterministic mode and the key query keep the
+    // reference order (ties by original id), which fixes its summation order.
+    const bool by_slot = reorder && s->p2g_mode == MPM_P2G_FUSED;
+    HIP_TRY(s, mpm::launch_tile_keys(a, s->keys, by_slot ? 1 : 0, s->stream));
     unsigned blocks = std::min<uint32_t>((s->n + 255) / 256, 8192u);
     hipLaunchKernelGGL(k_iota, dim3(std::max(blocks, 1u)), dim3(256), 0, s->stream, s->vals, s->n);
     HIP_TRY(s, hipGetLastError());
@@ -371,12 +381,16 @@ int sort_particles(mpm_sim* s, bool reorder) {
     HIP_TRY(s, mpm::radix_sort_pairs(s->keys, s->vals, s->n, bits, s->sort_scratch, s->stream));
     if (reorder) {
         const int other = 1 - s->ps_cur;
-        HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
-        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->src_slot, s->n, s->stream));
+        const uint32_t* src = s->vals;  // sorted previous slots
+        if (!by_slot) {
+            HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
+            src = s->src_slot;
+        }
+        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, src, s->n, s->stream));
         HIP_TRY(s, mpm::launch_index_slots(s->ps[other], s->cap, s->slot_of_id, s->n, s->stream));
         HIP_TRY(s, mpm::launch_static_rebuild(s->d_scenes, s->tile_scene, s->tile_flag, s->slist,
                                               s->status + mpm::ST_SCOUNT, s->keys, s->n, s->tile_lb,
-                                              s->total_tiles, grid_blocks(s), s->stream));
+                                              s->total_tiles, s->smark, s->spos, s->sscan, s->stream));
         HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
                                             s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
                                             s->chunk_of_slot, s->stream));

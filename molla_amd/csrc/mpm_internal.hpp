@@ -143,13 +143,16 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
                               float4* keep = nullptr);
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also = nullptr);
-// Static active list of a sort: the 3x3x3 dilation of occupied tiles.
+// Static active list of a sort: the 3x3x3 dilation of occupied tiles, in
+// ascending tile order (mark/pos: total_tiles u32 each; scan_scratch:
+// scan_scratch_words(total_tiles)).  Also builds the tile lower-bound table.
 hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_scene,
                                  uint32_t* tile_flag, uint32_t* slist, uint32_t* scount,
                                  const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_lb,
-                                 uint32_t total_tiles, int blocks, hipStream_t s);
+                                 uint32_t total_tiles, uint32_t* mark, uint32_t* pos, uint32_t* scan_scratch,
+                                 hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t s);
+hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_out, int by_slot, hipStream_t s);
 hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
                           uint32_t n, hipStream_t s);
 hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,

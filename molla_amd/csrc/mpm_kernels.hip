@@ -477,37 +477,6 @@ hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t*
 }
 
 // ---- static active list of a sort ----
-__global__ void k_static_clear(uint32_t* __restrict__ flag, const uint32_t* __restrict__ slist,
-                               const uint32_t* __restrict__ scount) {
-    const uint32_t n = *scount;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        flag[slist[i]] &= ~FLAG_STATIC;
-}
-
-// Every tile within one tile of an occupied tile (in each axis) may receive
-// P2G mass before the next sort.
-__global__ void k_static_mark(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
-                              const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ flag,
-                              uint32_t* __restrict__ slist, uint32_t* __restrict__ scount) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        const uint32_t t = keys[j];
-        if (j != 0 && keys[j - 1] == t) continue;
-        const SceneDev sc = scenes[tile_scene[t]];
-        const uint32_t tl = t - sc.tile_base;
-        const int W = int(sc.W);
-        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
-        for (int dz = -1; dz <= 1; ++dz)
-            for (int dy = -1; dy <= 1; ++dy)
-                for (int dxx = -1; dxx <= 1; ++dxx) {
-                    const int nx = tx + dxx, ny = ty + dy, nz = tz + dz;
-                    if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) continue;
-                    const uint32_t u = sc.tile_base + uint32_t(nx + W * (ny + W * nz));
-                    if (flag[u] & FLAG_STATIC) continue;
-                    if ((atomicOr(&flag[u], FLAG_STATIC) & FLAG_STATIC) == 0u) slist[atomicAdd(scount, 1u)] = u;
-                }
-    }
-}
-
 // tile_lb[k] = first sorted slot whose key is >= k, for k in [0, total].
 __global__ void k_tile_lb(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ lb,
                           uint32_t total) {
@@ -522,19 +491,52 @@ __global__ void k_tile_lb(const uint32_t* __restrict__ keys, uint32_t n, uint32_
     }
 }
 
+// A tile is on the static list when any tile within one tile of it (each
+// axis, same scene) holds a particle: P2G can reach it before the next sort.
+// Occupancy of a 3-tile x-run is one difference of the lower-bound table.
+__global__ void k_static_flags(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                               const uint32_t* __restrict__ lb, uint32_t total, uint32_t* __restrict__ flag,
+                               uint32_t* __restrict__ mark) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const SceneDev& sc = scenes[tile_scene[t]];
+        const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
+        bool on = false;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int ny = ty + dy, nz = tz + dz;
+                if (ny < 0 || nz < 0 || ny >= W || nz >= W) continue;
+                const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));
+                on = on || lb[row + xb + 1] > lb[row + xa];
+            }
+        mark[t] = on ? 1u : 0u;
+        flag[t] = (flag[t] & ~FLAG_STATIC) | (on ? FLAG_STATIC : 0u);
+    }
+}
+
+__global__ void k_static_emit(const uint32_t* __restrict__ mark, const uint32_t* __restrict__ pos, uint32_t total,
+                              uint32_t* __restrict__ slist, uint32_t* __restrict__ scount) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        if (mark[t]) slist[pos[t]] = t;
+        if (t + 1 == total) *scount = pos[t] + mark[t];
+    }
+}
+
 hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_scene,
                                  uint32_t* tile_flag, uint32_t* slist, uint32_t* scount,
                                  const uint32_t* sorted_keys, uint32_t n, uint32_t* tile_lb,
-                                 uint32_t total_tiles, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_static_clear, dim3(blocks), dim3(256), 0, s, tile_flag, slist, scount);
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, scount, 0u);
-    if (n > 0) {
-        unsigned nb = std::min<unsigned>((n + 255) / 256, 8192u);
-        hipLaunchKernelGGL(k_static_mark, dim3(nb), dim3(256), 0, s, scenes, tile_scene, sorted_keys, n, tile_flag,
-                           slist, scount);
-    }
+                                 uint32_t total_tiles, uint32_t* mark, uint32_t* pos, uint32_t* scan_scratch,
+                                 hipStream_t s) {
     unsigned lbb = std::min<unsigned>((total_tiles + 256) / 256, 16384u);
     hipLaunchKernelGGL(k_tile_lb, dim3(lbb), dim3(256), 0, s, sorted_keys, n, tile_lb, total_tiles);
+    const unsigned tb = std::max(1u, std::min<unsigned>((total_tiles + 255) / 256, 16384u));
+    hipLaunchKernelGGL(k_static_flags, dim3(tb), dim3(256), 0, s, scenes, tile_scene, tile_lb, total_tiles, tile_flag,
+                       mark);
+    hipError_t e = exclusive_scan_u32(mark, pos, total_tiles, scan_scratch, nullptr, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_static_emit, dim3(tb), dim3(256), 0, s, mark, pos, total_tiles, slist, scount);
     return hipGetLastError();
 }
 
@@ -555,8 +557,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 // Tile keys (shaders/mpm_particle_order.wgsl:13-25): cell = floor((x - o) *
 // inv_dx) - window, clamped to [0, G-1], /4, linearised x-fastest; the global
 // key adds the scene's tile base.  Written at the particle's original id so
-// that a stable sort over ids 0..n-1 reproduces the reference order.
-__global__ __launch_bounds__(256) void k_tile_keys(StepArgs a, uint32_t* __restrict__ keys_by_id) {
+// that a stable sort over ids 0..n-1 reproduces the reference order, or (the
+// layout sort) at the particle's slot, so ties keep the current slot order
+// and the reorder gather reads nearly sequential memory.
+__global__ __launch_bounds__(256) void k_tile_keys(StepArgs a, uint32_t* __restrict__ keys_out, int by_slot) {
     for (uint32_t ci = blockIdx.x; ci < a.n_chunks; ci += gridDim.x) {
         const Chunk ch = a.chunks[ci];
         if (threadIdx.x >= ch.count) continue;
@@ -570,15 +574,15 @@ __global__ __launch_bounds__(256) void k_tile_keys(StepArgs a, uint32_t* __restr
             ci2 = min(max(ci2, 0), int(sc.G) - 1);
             t[k] = uint32_t(ci2) >> 2;
         }
-        const uint32_t id = __float_as_uint(a.ps[(size_t)CID * a.cap + slot]);
-        keys_by_id[id] = sc.tile_base + t[0] + sc.W * (t[1] + sc.W * t[2]);
+        const uint32_t at = by_slot ? slot : __float_as_uint(a.ps[(size_t)CID * a.cap + slot]);
+        keys_out[at] = sc.tile_base + t[0] + sc.W * (t[1] + sc.W * t[2]);
     }
 }
 
-hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_by_id, hipStream_t s) {
+hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_out, int by_slot, hipStream_t s) {
     if (a.n_chunks == 0) return hipSuccess;
     unsigned blocks = a.n_chunks < 65535u ? a.n_chunks : 65535u;
-    hipLaunchKernelGGL(k_tile_keys, dim3(blocks), dim3(256), 0, s, a, keys_by_id);
+    hipLaunchKernelGGL(k_tile_keys, dim3(blocks), dim3(256), 0, s, a, keys_out, by_slot);
     return hipGetLastError();
 }
 
