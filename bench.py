@@ -196,6 +196,54 @@ def run_adjoint(scale):
             "loss": J, "dJ_dyoungs_sum": float(np.sum(g["E"], dtype=np.float64))}
 
 
+def run_batched_adjoint(scale):
+    """Batched system identification: 128 CPU-diff scenes (~1.8M particles,
+    48^3..80^3 grids), 32 substeps, checkpoint every 16; loss = sum over
+    scenes of the mean squared distance of x_T to the ground-truth run's."""
+    from molla_amd import MpmSimulation, workloads
+
+    n_scenes = max(4, int(128 * scale))
+    cfgs, d, meta = workloads.batched_diff_scenes(n_scenes)
+    counts, dt, steps, every = meta["counts"], meta["dt"], meta["substeps"], meta["checkpoint_every"]
+    n = int(sum(counts))
+    sim = MpmSimulation(cfgs)
+
+    def start(e_scene):
+        sim.set_particles_batch(d["x"], d["v"])
+        sim.write_field("youngs", np.concatenate([np.full(c, e, np.float32) for c, e in zip(counts, e_scene)]))
+
+    start(meta["e_true"])
+    sim.step_n(dt, steps)
+    target = sim.read_positions()
+    scale_p = np.repeat(1.0 / np.asarray(counts, np.float64), counts)[:, None]
+
+    def loss_grad():
+        r = sim.read_positions().astype(np.float64) - target
+        return float(np.sum(scale_p * r * r)), (2.0 * scale_p * r).astype(np.float32)
+
+    start(meta["e0"])
+    sim.diff_forward(dt, steps, every)
+    sim.diff_backward(loss_grad()[1])
+    start(meta["e0"])
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.diff_forward(dt, steps, every)
+    t_fwd = time.perf_counter() - t0
+    J, gx = loss_grad()
+    t0 = time.perf_counter()
+    g = sim.diff_backward(gx)
+    t_bwd = time.perf_counter() - t0
+    sim.close()
+    ge = g["youngs_per_scene"]
+    # the loss is minimised at E_true: dJ/dE should point from E0 towards it
+    agree = float(np.mean(np.sign(ge) == np.sign(np.asarray(meta["e0"]) - np.asarray(meta["e_true"]))))
+    return {"workload": "batched_diff_scenes", "scenes": n_scenes, "particles": n, "substeps": steps,
+            "checkpoint_every": every, "grids": [meta["grid_min"], meta["grid_max"]],
+            "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd, "adjoint_fwd_ratio": t_bwd / t_fwd,
+            "forward_value": n * steps / t_fwd, "unit": "particle-substeps/s", "loss": J,
+            "dJ_dE_sign_agrees_with_E0_minus_Etrue": agree}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,6 +327,11 @@ def main():
             log(f"adjoint: {out['adjoint']}")
         except Exception as e:  # reported, never silently replaced
             out["adjoint"] = {"error": str(e)}
+        try:
+            out["batched_adjoint"] = run_batched_adjoint(args.scale)
+            log(f"batched adjoint: {out['batched_adjoint']}")
+        except Exception as e:  # reported, never silently replaced
+            out["batched_adjoint"] = {"error": str(e)}
 
     if not args.no_cpu:
         sim.close()

@@ -219,6 +219,7 @@ mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
     a.scenes = s->d_scenes;
     a.mats = s->d_mats;
     a.ps = s->ps[s->ps_cur];
+    a.ps_out = a.ps;
     a.cap = s->cap;
     a.gmp = s->gmp;
     a.gv = s->gv;
@@ -416,7 +417,7 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = null
     if (s->dry) return MPM_OK;
     Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
     mpm::StepArgs a = args(s, dt, list_out);
-    if (ps) a.ps = ps;
+    if (ps) a.ps = a.ps_out = ps;
     int blocks = particle_blocks(s);
     if (s->p2g_mode == MPM_P2G_FUSED && s->sorted) use_sorted_chunks(s, a, blocks);
     HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, blocks, s->stream));
@@ -723,7 +724,7 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
 mpm::AdjArgs adj_args(mpm_sim* s, float dt, int list, float* ps, int& blocks) {
     mpm::AdjArgs q{};
     q.a = args(s, dt, list);
-    q.a.ps = ps;
+    q.a.ps = q.a.ps_out = ps;
     q.a.gmp = s->gadj;  // the reverse G2P's escaped particles accumulate here
     q.a.use_lds = 1;
     use_sorted_chunks(s, q.a, blocks);
@@ -781,12 +782,15 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     int L = s->list_cur;
     auto count = [&](int l) { return s->status + mpm::ST_COUNT0 + l; };
     const uint32_t nseg = (steps + every - 1) / every;
+    // planes 24.. (jp, mass, volume, E, material, id) do not change in the
+    // CPU-diff model: every tape slot gets them once
+    for (uint32_t k = 0; k < every; ++k)
+        HIP_TRY(s, hipMemcpyAsync(s->tape + k * st_words + 24 * plane, s->ps[s->ps_cur] + 24 * plane,
+                                  (size_t)(mpm::NCOMP - 24) * plane * 4, hipMemcpyDeviceToDevice, s->stream));
     for (int seg = int(nseg) - 1; seg >= 0 && rc == MPM_OK; --seg) {
         const uint32_t len = std::min(every, steps - uint32_t(seg) * every);
         float* t0 = s->tape;
         HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
-        HIP_TRY(s, hipMemcpyAsync(t0 + 24 * plane, s->ps[s->ps_cur] + 24 * plane, (size_t)(mpm::NCOMP - 24) * plane * 4,
-                                  hipMemcpyDeviceToDevice, s->stream));
         // Recompute the segment with the fused forward: substep k's grid goes
         // to the grid tape and its pre-state to tape slot k.
         if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0)) != MPM_OK) break;
@@ -798,9 +802,9 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
                                                s->stream, gk.gv + cells));
             if (k + 1 == len) break;
             float* tk1 = tk + st_words;
-            HIP_TRY(s, hipMemcpyAsync(tk1, tk, st_words * 4, hipMemcpyDeviceToDevice, s->stream));
             mpm::StepArgs a = args(s, dt, 1 - L);
-            a.ps = tk1;
+            a.ps = tk;
+            a.ps_out = tk1;  // state k+1 goes straight to the next tape slot
             a.gv = gk.gv;
             a.write_vc = 1;
             int blocks = 0;

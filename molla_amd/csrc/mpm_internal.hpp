@@ -112,7 +112,9 @@ struct GridDev {
 struct StepArgs {
     const SceneDev* scenes;
     const float* mats;
-    float* ps;  // particle planes
+    float* ps;      // particle planes read by the pass
+    float* ps_out;  // where a G2P pass writes x, v, C, F, jp (ps itself, or the
+                    // next tape slot of the adjoint's recompute)
     uint32_t cap;
     float4* gmp;
     float4* gv;

@@ -67,7 +67,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
     if (L.box[15]) return;  // block-uniform: a latched fault stops the pass
     if (MODE != MODE_P2G && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.status[ST_GATHERS], 1u);
     const uint32_t cap = a.cap;
-    float* __restrict__ ps = a.ps;
+    const float* ps = a.ps;
+    float* po = a.ps_out;  // the updated x, v, C, F, jp (may alias ps)
     const float dt = a.dt;
     const uint32_t nch = a.n_chunks_dev ? *a.n_chunks_dev : a.n_chunks;
     for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
@@ -303,16 +304,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                         }
                         // ---- write the particle state ----
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) ps[(size_t)(CX + k) * cap + slot] = nx[k];
+                        for (int k = 0; k < 3; ++k) po[(size_t)(CX + k) * cap + slot] = nx[k];
                         if (a.write_vc || MODE == MODE_G2P) {
 #pragma unroll
-                            for (int k = 0; k < 3; ++k) ps[(size_t)(CV + k) * cap + slot] = vn[k];
+                            for (int k = 0; k < 3; ++k) po[(size_t)(CV + k) * cap + slot] = vn[k];
 #pragma unroll
-                            for (int k = 0; k < 9; ++k) ps[(size_t)(CC + k) * cap + slot] = Cn[k];
+                            for (int k = 0; k < 9; ++k) po[(size_t)(CC + k) * cap + slot] = Cn[k];
                         }
 #pragma unroll
-                        for (int k = 0; k < 9; ++k) ps[(size_t)(CF + k) * cap + slot] = Fn[k];
-                        if (MODEL == MODEL_SHADER && plast_ok) ps[(size_t)CJP * cap + slot] = jp_new;
+                        for (int k = 0; k < 9; ++k) po[(size_t)(CF + k) * cap + slot] = Fn[k];
+                        if (MODEL == MODEL_SHADER && plast_ok) po[(size_t)CJP * cap + slot] = jp_new;
                         if (MODE == MODE_G2P2G && plast_ok) {
                             if (MODEL == MODEL_CPUDIFF) svd3<MODEL_CPUDIFF>(Fn, d);
                             (void)have_svd;

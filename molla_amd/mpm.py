@@ -387,8 +387,12 @@ class MpmSimulation:
                                             _ptr(oe), _ptr(ml)))
         nu = float(self.cfgs[0].poisson)
         g_youngs = ml[0] / (2 * (1 + nu)) + ml[1] * nu / ((1 + nu) * (1 - 2 * nu))
+        # per-scene dJ/dE (scene-wide E): the sum of the scene's per-particle dJ/dE_p
+        bounds = np.concatenate([[0], np.cumsum(self.counts)]).astype(np.int64)
+        e_scene = np.array([np.sum(oe[bounds[i]:bounds[i + 1]], dtype=np.float64)
+                            for i in range(len(self.counts))])
         return {"x": ox, "v": ov, "E": oe, "mu": float(ml[0]), "lambda": float(ml[1]),
-                "youngs": float(g_youngs)}
+                "youngs": float(g_youngs), "youngs_per_scene": e_scene}
 
     def read_adjoint(self, name: str):
         """dJ/d(initial field) of the last diff_backward, original particle order."""

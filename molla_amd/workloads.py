@@ -138,3 +138,40 @@ def diff_scene(n: int = 65_536, seed: int = 4, grid: int = 64) -> Tuple[MpmConfi
             "checkpoint_every": 16, "rho": rho, "seed": seed, "target": target.tolist(),
             "cfl": dt * _cp(2e4, 0.3, rho) / dx}
     return cfg, {"x": x, "v": v, "E": E, "target": target}, meta
+
+
+def batched_diff_scenes(n_scenes: int = 128, seed: int = 5, n_lo: float = 4e3, n_hi: float = 3.2e4,
+                        g_lo: int = 48, g_hi: int = 80, rho: float = 1000.0,
+                        dt: float = 5e-4) -> Tuple[List[MpmConfig], Dict[str, List[np.ndarray]], Dict]:
+    """Batched system identification (CPU-diff semantics): per scene an
+    elastic block of N ~ logU[4k, 32k] particles dropped onto the floor of a
+    48^3..80^3 grid; the ground-truth Young's modulus E_true ~ logU[5e3, 2e4]
+    per scene and the starting guess E_0 = E_true * exp(U[-0.3, 0.3])."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_scenes))).astype(int)
+    t = (np.log(counts) - math.log(n_lo)) / (math.log(n_hi) - math.log(n_lo))
+    cfgs, xs, vs, e_true, e0 = [], [], [], [], []
+    cfl = 0.0
+    for i, N in enumerate(counts):
+        G = int(4 * round((g_lo + (g_hi - g_lo) * t[i]) / 4))
+        dx = 1.0 / G
+        side = (N / 8.0) ** (1.0 / 3.0) * dx  # 8 particles per cell
+        lo = np.array([0.5 - side / 2, 3.5 * dx, 0.5 - side / 2])
+        x = (lo + side * rng.random((N, 3))).astype(np.float32)
+        v = np.tile(np.array([rng.uniform(-0.3, 0.3), -1.0, rng.uniform(-0.3, 0.3)], np.float32), (N, 1))
+        vol = side ** 3 / N
+        et = float(math.exp(rng.uniform(math.log(5e3), math.log(2e4))))
+        cfgs.append(MpmConfig(grid_size=G, cell_size=dx, grid_origin=(0.0, 0.0, 0.0), gravity_y=-9.8,
+                              youngs=et, poisson=0.3, particle_mass=rho * vol, particle_volume=vol,
+                              max_particles=int(N), model=MODEL_CPUDIFF, floor_y=3 * dx, floor_friction=0.5,
+                              rheology=ELASTIC))
+        xs.append(x)
+        vs.append(v)
+        e_true.append(et)
+        e0.append(et * math.exp(rng.uniform(-0.3, 0.3)))
+        cfl = max(cfl, dt * _cp(2e4, 0.3, rho) / dx)
+    meta = {"workload": "batched_diff_scenes", "scenes": n_scenes, "particles": int(counts.sum()),
+            "counts": counts.tolist(), "dt": dt, "substeps": 32, "checkpoint_every": 16, "rho": rho,
+            "seed": seed, "cfl_max": cfl, "e_true": e_true, "e0": e0,
+            "grid_min": int(min(c.grid_size for c in cfgs)), "grid_max": int(max(c.grid_size for c in cfgs))}
+    return cfgs, {"x": xs, "v": vs}, meta

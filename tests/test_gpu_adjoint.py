@@ -164,3 +164,37 @@ def test_centroid_loss_gradient_matches_gpu_finite_difference():
     an_e = float(np.sum(g["E"].astype(np.float64) * d["E"] * dE))
     log(f"  dJ/dE . (E dE): adjoint {an_e:.6e} fd {fd_e:.6e}")
     assert abs(an_e - fd_e) <= 5e-2 * abs(fd_e) + 1e-9
+
+
+def test_batched_adjoint_matches_oracle_per_scene():
+    # System identification on a batch: each scene's reverse pass against the
+    # oracle run on that scene alone, and the per-scene dJ/dE sums.
+    cfgs, d, meta = workloads.batched_diff_scenes(3, seed=31, n_lo=1000, n_hi=3000, g_lo=32, g_hi=32)
+    dt, steps = meta["dt"], 16
+    counts = meta["counts"]
+    n = int(sum(counts))
+    rng = np.random.default_rng(32)
+    Es = [np.full(c, e, np.float32) for c, e in zip(counts, meta["e0"])]
+    Fs = [(np.eye(3)[None] + 0.03 * rng.standard_normal((c, 3, 3))).astype(np.float32) for c in counts]
+    gx = rng.standard_normal((n, 3)).astype(np.float32)
+    sim = MpmSimulation(cfgs)
+    sim.set_particles_batch(d["x"], d["v"])
+    sim.write_field("F", np.concatenate(Fs).reshape(-1, 9))
+    sim.write_field("youngs", np.concatenate(Es))
+    sim.diff_forward(dt, steps, 8)
+    g = sim.diff_backward(gx)
+    gF = sim.read_adjoint("F").reshape(-1, 3, 3)
+    sim.close()
+    off = 0
+    for i, c in enumerate(counts):
+        sl = slice(off, off + c)
+        dd = {"x": d["x"][i], "v": d["v"][i], "E": Es[i], "F": Fs[i]}
+        o = _oracle_run(cfgs[i], dd, dt, steps, gx[sl], np.zeros((c, 3), np.float32))
+        errs = {"x": rel_err(g["x"][sl] - gx[sl], o["x"] - gx[sl]), "v": rel_err(g["v"][sl], o["v"]),
+                "F": rel_err(gF[sl], o["F"]), "E": rel_err(g["E"][sl], o["E"])}
+        es = abs(g["youngs_per_scene"][i] - o["E"].sum()) / max(abs(o["E"].sum()), 1e-30)
+        log(f"  scene {i}: n={c} G={cfgs[i].grid_size} errs {errs} dJ/dE_scene rel {es:.2e}")
+        for k, e in errs.items():
+            assert e < 1e-4, (i, k)
+        assert es < 1e-3
+        off += c
