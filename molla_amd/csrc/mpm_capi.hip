@@ -789,6 +789,10 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
                                   (size_t)(mpm::NCOMP - 24) * plane * 4, hipMemcpyDeviceToDevice, s->stream));
     for (int seg = int(nseg) - 1; seg >= 0 && rc == MPM_OK; --seg) {
         const uint32_t len = std::min(every, steps - uint32_t(seg) * every);
+        uint32_t w0[mpm::ST_WORDS];
+        HIP_TRY(s, hipMemcpyAsync(w0, s->status, sizeof(w0), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        const uint32_t esc0 = w0[mpm::ST_ESCAPES], fb0 = w0[mpm::ST_FALLBACK];
         float* t0 = s->tape;
         HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
         // Recompute the segment with the fused forward: substep k's grid goes
@@ -816,16 +820,47 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
         // The last grid update reset the other list; the reverse builds its
         // own list of escaped tiles there.
         const int R = 1 - L;
+        // No particle of the segment scattered with global atomics (its reverse
+        // scatters use the same stencils): the reverse P2G of substep k and the
+        // reverse G2P of substep k-1 run as one pass.
+        uint32_t w1[mpm::ST_WORDS];
+        HIP_TRY(s, hipMemcpyAsync(w1, s->status, sizeof(w1), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        const bool fused = w1[mpm::ST_ESCAPES] == esc0 && w1[mpm::ST_FALLBACK] == fb0;
+        auto tape_at = [&](int k) { return s->tape + (size_t)k * st_words; };
+        auto gv_at = [&](int k) { return s->gtape + (size_t)2 * k * cells; };
+        if (fused) {
+            int blocks = 0;
+            mpm::AdjArgs q = adj_args(s, dt, R, tape_at(int(len) - 1), blocks);
+            q.a.gv = gv_at(int(len) - 1);
+            HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
+            for (int k = int(len) - 1; k >= 0; --k) {
+                mpm::GridDev gk = g;
+                gk.gmp = gv_at(k) + cells;
+                HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 0, gb,
+                                                s->stream));
+                mpm::AdjArgs qk = adj_args(s, dt, R, tape_at(k), blocks);
+                if (k > 0)
+                    HIP_TRY(s, mpm::launch_adj_fused(qk, tape_at(k - 1), gv_at(k - 1), blocks, s->stream));
+                else
+                    HIP_TRY(s, mpm::launch_adj_p2g(qk, blocks, s->stream));
+            }
+            mpm::GridDev gc = g;
+            gc.gmp = s->gadj;
+            HIP_TRY(s, mpm::launch_clear_tiles(gc, s->list[R], count(R), gb, s->stream));
+            L = R;
+            continue;
+        }
         for (int k = int(len) - 1; k >= 0 && rc == MPM_OK; --k) {
-            float* tk = s->tape + (size_t)k * st_words;
-            float4* gvk = s->gtape + (size_t)2 * k * cells;
+            float* tk = tape_at(k);
+            float4* gvk = gv_at(k);
             int blocks = 0;
             mpm::AdjArgs q = adj_args(s, dt, R, tk, blocks);
             q.a.gv = gvk;
             HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
             mpm::GridDev gk = g;
             gk.gmp = gvk + cells;
-            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, gb, s->stream));
+            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 1, gb, s->stream));
             HIP_TRY(s, mpm::launch_adj_p2g(q, blocks, s->stream));
             mpm::GridDev gc = g;
             gc.gmp = s->gadj;

@@ -6,7 +6,8 @@
 //   P2G (mass, momentum + stress) -> grid (v = p/m, gravity, floor)
 //   -> G2P (v, C) -> x += dt v, F = (I + dt C) F,
 // and its reverse runs as three launches over the substep's pre-state, which
-// the host recomputes from a checkpoint:
+// the host recomputes from a checkpoint (k_adj_fused merges the reverse P2G
+// of one substep with the reverse G2P of the previous one):
 //   k_adj_g2p   reverse of G2P and of the x/F advance.  Gathers grid v like
 //               G2P and scatters dJ/d(grid v) = sum w (gv' + 4/dx^2 gC' d)
 //               with the forward's block scatter (the same form as P2G's
@@ -55,14 +56,334 @@ __device__ __forceinline__ double warp_sum(double v) {
 
 }  // namespace
 
+// Reverse of G2P and of the x/F advance for the particle in `slot` of the
+// pre-state a.ps with grid velocity a.gv (mpm_adjoint.rs:196-305).  On entry
+// gx, gv, gC, gF are dJ/d(post-state); on return gx holds the G2P part of
+// dJ/dx (plus the pass-through), gF = (I + dt C')^T dJ/dF', and sp the
+// scatter of dJ/d(grid v).
+__device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneDev& sc, uint32_t slot, float* gx,
+                                                 const float* gv, const float* gC, float* gF, Scatter& sp) {
+    const uint32_t cap = a.cap;
+    const float* __restrict__ ps = a.ps;
+    const float dt = a.dt;
+    float x[3], F[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = ps[(size_t)(CX + k) * cap + slot];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = ps[(size_t)(CF + k) * cap + slot];
+    // x' = x + dt v', F' = (I + dt C') F  =>  dJ/dv' += dt dJ/dx',
+    // dJ/dC' += dt dJ/dF' F^T (mpm_adjoint.rs:196-247)
+    float gvp[3], B[9];
+    const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gvp[k] = gx[k] * dt + gv[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[3 * i + j] = s4 * (gC[3 * i + j] + dt * (gF[3 * i] * F[3 * j] + gF[3 * i + 1] * F[3 * j + 1] +
+                                                       gF[3 * i + 2] * F[3 * j + 2]));
+    float xp[3], fx[3], w[3][3], dw[3][3];
+    int b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
+        const float bf = floorf(xp[k] - 0.5f);
+        b[k] = int(bf);
+        fx[k] = xp[k] - bf;
+        quad_weights(fx[k], w[k]);
+        dw[k][0] = -(1.5f - fx[k]);
+        dw[k][1] = -2.0f * (fx[k] - 1.0f);
+        dw[k][2] = fx[k] - 0.5f;
+    }
+    // G2P reverse (mpm_adjoint.rs:249-305): with B = 4/dx^2 dJ/dC',
+    //   dJ/dx += sum grad(w) (v_i . (dJ/dv' + B d)) - B^T sum w v_i
+    const float dx = sc.dx, idx = sc.inv_dx;
+    float vn[3] = {0.f, 0.f, 0.f}, Cn[9], acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Cn[k] = 0.f;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
+                if (!in_grid(sc, ix, iy, iz)) continue;
+                const float4 g = a.gv[cell_index(sc, ix, iy, iz)];
+                const float wt = w[0][i] * w[1][j] * w[2][k];
+                const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
+                            d2 = (float(k) - fx[2]) * dx;
+                vn[0] += wt * g.x;
+                vn[1] += wt * g.y;
+                vn[2] += wt * g.z;
+                const float sw = s4 * wt;
+                const float gx0 = sw * g.x, gy0 = sw * g.y, gz0 = sw * g.z;
+                Cn[0] += gx0 * d0; Cn[1] += gx0 * d1; Cn[2] += gx0 * d2;
+                Cn[3] += gy0 * d0; Cn[4] += gy0 * d1; Cn[5] += gy0 * d2;
+                Cn[6] += gz0 * d0; Cn[7] += gz0 * d1; Cn[8] += gz0 * d2;
+                const float h = g.x * (gvp[0] + B[0] * d0 + B[1] * d1 + B[2] * d2) +
+                                g.y * (gvp[1] + B[3] * d0 + B[4] * d1 + B[5] * d2) +
+                                g.z * (gvp[2] + B[6] * d0 + B[7] * d1 + B[8] * d2);
+                acc[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
+                acc[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
+                acc[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        acc[k] -= B[k] * vn[0] + B[3 + k] * vn[1] + B[6 + k] * vn[2];
+        gx[k] += acc[k];
+    }
+    // dJ/dF (pre) = (I + dt C')^T dJ/dF'
+    float nF[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            nF[3 * i + j] = gF[3 * i + j] + dt * (Cn[i] * gF[j] + Cn[3 + i] * gF[3 + j] + Cn[6 + i] * gF[6 + j]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gF[k] = nF[k];
+    // dJ/d(grid v_i) += w (dJ/dv' + B d): the P2G form with m = 0
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sp.b[k] = b[k];
+        sp.fx[k] = fx[k];
+        sp.mv[k] = gvp[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sp.A[k] = B[k];
+    sp.m = 0.f;
+    sp.on = true;
+    sp.esc = stencil_escapes(a, sc, slot, sp.b);
+}
+
+// Reverse of P2G, of the stress and of the SVD for the particle in `slot`
+// of the pre-state a.ps, gathering (dJ/dm, dJ/dp) from gadj
+// (mpm_adjoint.rs:389-487, mpm_cpu_diff.rs:330-379, dense3.rs:321-372).  On
+// entry gx, gF hold the reverse G2P's output; on return gx, gv, gC, gF are
+// dJ/d(pre-state), gE has dJ/dE_p added, and (amu, alam) are the particle's
+// dJ/dmu, dJ/dlambda.
+__device__ __forceinline__ void adj_p2g_particle(const StepArgs& a, const float4* __restrict__ gadj,
+                                                 const SceneDev& sc, uint32_t slot, float* gx, float* gF, float* gv,
+                                                 float* gC, float& gE, double& amu, double& alam) {
+    const uint32_t cap = a.cap;
+    const float* __restrict__ ps = a.ps;
+    const float dt = a.dt;
+    float x[3], v[3], C[9], F[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        x[k] = ps[(size_t)(CX + k) * cap + slot];
+        v[k] = ps[(size_t)(CV + k) * cap + slot];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        C[k] = ps[(size_t)(CC + k) * cap + slot];
+        F[k] = ps[(size_t)(CF + k) * cap + slot];
+    }
+    const float m = ps[(size_t)CMASS * cap + slot];
+    const float vol = ps[(size_t)CVOL * cap + slot];
+    float mu = sc.mu, lam = sc.lam;
+    if (a.has_E) {
+        const float E = ps[(size_t)CE * cap + slot];
+        mu = E / (2.0f * (1.0f + sc.poisson));
+        lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
+    }
+    // corotated-SVK PK1 from the FP64 SVD (mpm_cpu_diff.rs:279-287)
+    double U[9], s[3], V[9];
+    svd3_f64_raw(F, U, s, V);
+    svd3_signs<MODEL_CPUDIFF>(U, s, V);
+    const double dmu = mu, dlam = lam;
+    const double trsi = s[0] + s[1] + s[2] - 3.0;
+    Dm3 P;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double r = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
+            const double fr = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                              U[3 * i + 2] * s[2] * V[3 * j + 2];
+            P(i, j) = 2.0 * dmu * (fr - r) + dlam * trsi * r;
+        }
+    // A = m C + 4/dx^2 stress, stress = -dt vol P F^T
+    const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
+    const double coef = -(double)dt * (double)vol;
+    float A[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double st = coef * (P(i, 0) * F[3 * j] + P(i, 1) * F[3 * j + 1] + P(i, 2) * F[3 * j + 2]);
+            A[3 * i + j] = m * C[3 * i + j] + s4 * (float)st;
+        }
+    float mv[3] = {m * v[0], m * v[1], m * v[2]};
+    float xp[3], fx[3], w[3][3], dw[3][3];
+    int b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
+        const float bf = floorf(xp[k] - 0.5f);
+        b[k] = int(bf);
+        fx[k] = xp[k] - bf;
+        quad_weights(fx[k], w[k]);
+        dw[k][0] = -(1.5f - fx[k]);
+        dw[k][1] = -2.0f * (fx[k] - 1.0f);
+        dw[k][2] = fx[k] - 0.5f;
+    }
+    // P2G reverse (mpm_adjoint.rs:389-487): with gw = sum w dJ/dp_i and
+    // T = sum w dJ/dp_i d^T:  dJ/dv = m gw, dJ/dC = m T,
+    // dJ/dstress = 4/dx^2 T, dJ/dx = sum grad(w) (dJ/dp_i . (m v + A d)
+    // + m dJ/dm_i) - A^T gw
+    const float dx = sc.dx, idx = sc.inv_dx;
+    float gw[3] = {0.f, 0.f, 0.f}, T[9], gpos[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) T[k] = 0.f;
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
+                if (!in_grid(sc, ix, iy, iz)) continue;
+                const float4 g = gadj[cell_index(sc, ix, iy, iz)];
+                const float wt = w[0][i] * w[1][j] * w[2][k];
+                const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
+                            d2 = (float(k) - fx[2]) * dx;
+                gw[0] += wt * g.y;
+                gw[1] += wt * g.z;
+                gw[2] += wt * g.w;
+                const float ax = wt * g.y, ay = wt * g.z, az = wt * g.w;
+                T[0] += ax * d0; T[1] += ax * d1; T[2] += ax * d2;
+                T[3] += ay * d0; T[4] += ay * d1; T[5] += ay * d2;
+                T[6] += az * d0; T[7] += az * d1; T[8] += az * d2;
+                const float h = g.y * (mv[0] + A[0] * d0 + A[1] * d1 + A[2] * d2) +
+                                g.z * (mv[1] + A[3] * d0 + A[4] * d1 + A[5] * d2) +
+                                g.w * (mv[2] + A[6] * d0 + A[7] * d1 + A[8] * d2) + m * g.x;
+                gpos[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
+                gpos[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
+                gpos[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
+            }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gpos[k] -= A[k] * gw[0] + A[3 + k] * gw[1] + A[6 + k] * gw[2];
+        gx[k] += gpos[k];
+        gv[k] = m * gw[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) gC[k] = m * T[k];
+    // stress = coef P F^T with dJ/dstress = s4 T:
+    //   dJ/dP = coef s4 T F,  dJ/dF += coef s4 T^T P
+    Dm3 aP, gf;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            aP(i, j) = coef * (double)s4 *
+                       ((double)T[3 * i] * F[j] + (double)T[3 * i + 1] * F[3 + j] + (double)T[3 * i + 2] * F[6 + j]);
+            gf(i, j) = (double)gF[3 * i + j] +
+                       coef * (double)s4 *
+                           ((double)T[i] * P(0, j) + (double)T[3 + i] * P(1, j) + (double)T[6 + i] * P(2, j));
+        }
+    // corotated_pk1_adjoint (mpm_cpu_diff.rs:330-379), in the rotated
+    // frame: with Ut aP V = Q, aU = aP V (2mu Sd + cr I) - ..., the SVD
+    // adjoint only needs Ut aU, Vt aV, so form them directly.
+    Dm3 Q;  // U^T aP V
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double pv = aP(r, 0) * V[j] + aP(r, 1) * V[3 + j] + aP(r, 2) * V[6 + j];
+                acc += U[3 * r + i] * pv;
+            }
+            Q(i, j) = acc;
+        }
+    const double cr = -2.0 * dmu + dlam * trsi;
+    const double r_dot = Q(0, 0) + Q(1, 1) + Q(2, 2);  // <R, aP> = tr(U^T aP V)
+    // aU = aP V (2mu Sd) + cr aP V  ->  Ut aU = Q (2mu Sd + cr I)
+    // aV = aP^T U (2mu Sd) + cr aP^T U  ->  Vt aV = Q^T (2mu Sd + cr I)
+    Dm3 UaU, VaV;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double dj = 2.0 * dmu * s[j] + cr;
+            UaU(i, j) = Q(i, j) * dj;
+            VaV(i, j) = Q(j, i) * dj;
+        }
+    double aS[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) aS[i] = 2.0 * dmu * Q(i, i) + dlam * r_dot;
+    // <Fr - R, aP> = sum_i (s_i - 1) Q_ii
+    amu = 2.0 * ((s[0] - 1.0) * Q(0, 0) + (s[1] - 1.0) * Q(1, 1) + (s[2] - 1.0) * Q(2, 2));
+    alam = trsi * r_dot;
+    // adj_svd3 (dense3.rs:321-372): dJ/dF += U M V^T,
+    //   M = diag(aS) + (f o om_u) Sd + Sd (f o om_v),
+    //   om_u = Ut aU - aU^T U = UaU - UaU^T, om_v = VaV - VaV^T
+    const double f01 = safe_inv_sq_diff(s[0], s[1]);
+    const double f02 = safe_inv_sq_diff(s[0], s[2]);
+    const double f12 = safe_inv_sq_diff(s[1], s[2]);
+    Dm3 Mx;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double f = 0.0;
+            if (i == 0 && j == 1) f = f01;
+            if (i == 0 && j == 2) f = f02;
+            if (i == 1 && j == 2) f = f12;
+            if (i == 1 && j == 0) f = -f01;
+            if (i == 2 && j == 0) f = -f02;
+            if (i == 2 && j == 1) f = -f12;
+            const double omu = UaU(i, j) - UaU(j, i);
+            const double omv = VaV(i, j) - VaV(j, i);
+            Mx(i, j) = (i == j ? aS[i] : 0.0) + f * omu * s[j] + s[i] * f * omv;
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = gf(i, j);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                acc += U[3 * i + r] * (Mx(r, 0) * V[3 * j] + Mx(r, 1) * V[3 * j + 1] + Mx(r, 2) * V[3 * j + 2]);
+            gF[3 * i + j] = (float)acc;
+        }
+    // dJ/dE_p (per-particle E, chain rule through lame, mpm_cpu_diff.rs:217-222)
+    const double nu = sc.poisson;
+    const double ge = amu / (2.0 * (1.0 + nu)) + alam * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    gE += (float)ge;
+}
+
+// Block sum of the particles' (dJ/dmu, dJ/dlambda) added to chunk ci's
+// accumulator: one block owns a chunk, so the order is fixed.  Every thread
+// of the block calls it.
+__device__ __forceinline__ void add_chunk_mu_lambda(double* gml, uint32_t ci, double amu, double alam,
+                                                    double (*red)[WAVES]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    amu = warp_sum(amu);
+    alam = warp_sum(alam);
+    if (lane == 0) {
+        red[0][wave] = amu;
+        red[1][wave] = alam;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sm = 0.0, sl = 0.0;
+        for (int w = 0; w < WAVES; ++w) {
+            sm += red[0][w];
+            sl += red[1][w];
+        }
+        gml[2 * (size_t)ci] += sm;
+        gml[2 * (size_t)ci + 1] += sl;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_adj_g2p(AdjArgs q) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_adj_g2p(AdjArgs q) {
     __shared__ ScatterLds L;
     const StepArgs& a = q.a;
     const uint32_t cap = a.cap;
-    const float* __restrict__ ps = a.ps;
     float* __restrict__ adj = q.adj;
-    const float dt = a.dt;
     const uint32_t nch = *a.n_chunks_dev;
     for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
         const uint32_t ci = w.ci;
@@ -73,101 +394,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_g2p(AdjArgs q) {
         sp.on = false;
         sp.esc = false;
         if (threadIdx.x < ch.count) {
-            float x[3], F[9], gx[3], gv[3], gC[9], gF[9];
+            float gx[3], gv[3], gC[9], gF[9];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                x[k] = ps[(size_t)(CX + k) * cap + slot];
                 gx[k] = adj[(size_t)(AX + k) * cap + slot];
                 gv[k] = adj[(size_t)(AV + k) * cap + slot];
             }
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                F[k] = ps[(size_t)(CF + k) * cap + slot];
                 gC[k] = adj[(size_t)(AC + k) * cap + slot];
                 gF[k] = adj[(size_t)(AF + k) * cap + slot];
             }
-            // x' = x + dt v', F' = (I + dt C') F  =>  dJ/dv' += dt dJ/dx',
-            // dJ/dC' += dt dJ/dF' F^T (mpm_adjoint.rs:196-247)
-            float gvp[3], B[9];
-            const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
+            adj_g2p_particle(a, sc, slot, gx, gv, gC, gF, sp);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) gvp[k] = gx[k] * dt + gv[k];
+            for (int k = 0; k < 3; ++k) adj[(size_t)(AX + k) * cap + slot] = gx[k];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    B[3 * i + j] = s4 * (gC[3 * i + j] + dt * (gF[3 * i] * F[3 * j] + gF[3 * i + 1] * F[3 * j + 1] +
-                                                               gF[3 * i + 2] * F[3 * j + 2]));
-            float xp[3], fx[3], w[3][3], dw[3][3];
-            int b[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
-                const float bf = floorf(xp[k] - 0.5f);
-                b[k] = int(bf);
-                fx[k] = xp[k] - bf;
-                quad_weights(fx[k], w[k]);
-                dw[k][0] = -(1.5f - fx[k]);
-                dw[k][1] = -2.0f * (fx[k] - 1.0f);
-                dw[k][2] = fx[k] - 0.5f;
-            }
-            // G2P reverse (mpm_adjoint.rs:249-305): with B = 4/dx^2 dJ/dC',
-            //   dJ/dx += sum grad(w) (v_i . (dJ/dv' + B d)) - B^T sum w v_i
-            const float dx = sc.dx, idx = sc.inv_dx;
-            float vn[3] = {0.f, 0.f, 0.f}, Cn[9], acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Cn[k] = 0.f;
-            for (int k = 0; k < 3; ++k)
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
-                        if (!in_grid(sc, ix, iy, iz)) continue;
-                        const float4 g = a.gv[cell_index(sc, ix, iy, iz)];
-                        const float wt = w[0][i] * w[1][j] * w[2][k];
-                        const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
-                                    d2 = (float(k) - fx[2]) * dx;
-                        vn[0] += wt * g.x;
-                        vn[1] += wt * g.y;
-                        vn[2] += wt * g.z;
-                        const float sw = s4 * wt;
-                        const float gx0 = sw * g.x, gy0 = sw * g.y, gz0 = sw * g.z;
-                        Cn[0] += gx0 * d0; Cn[1] += gx0 * d1; Cn[2] += gx0 * d2;
-                        Cn[3] += gy0 * d0; Cn[4] += gy0 * d1; Cn[5] += gy0 * d2;
-                        Cn[6] += gz0 * d0; Cn[7] += gz0 * d1; Cn[8] += gz0 * d2;
-                        const float h = g.x * (gvp[0] + B[0] * d0 + B[1] * d1 + B[2] * d2) +
-                                        g.y * (gvp[1] + B[3] * d0 + B[4] * d1 + B[5] * d2) +
-                                        g.z * (gvp[2] + B[6] * d0 + B[7] * d1 + B[8] * d2);
-                        acc[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
-                        acc[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
-                        acc[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
-                    }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                acc[k] -= B[k] * vn[0] + B[3 + k] * vn[1] + B[6 + k] * vn[2];
-                adj[(size_t)(AX + k) * cap + slot] = gx[k] + acc[k];
-            }
-            // dJ/dF (pre) = (I + dt C')^T dJ/dF'
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j)
-                    adj[(size_t)(AF + 3 * i + j) * cap + slot] =
-                        gF[3 * i + j] + dt * (Cn[i] * gF[j] + Cn[3 + i] * gF[3 + j] + Cn[6 + i] * gF[6 + j]);
-            // dJ/d(grid v_i) += w (dJ/dv' + B d): the P2G form with m = 0
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                sp.b[k] = b[k];
-                sp.fx[k] = fx[k];
-                sp.mv[k] = gvp[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) sp.A[k] = B[k];
-            sp.m = 0.f;
-            sp.on = true;
-            sp.esc = stencil_escapes(a, sc, slot, sp.b);
+            for (int k = 0; k < 9; ++k) adj[(size_t)(AF + k) * cap + slot] = gF[k];
         }
-        block_scatter(a, sc, sp, ci, L);
+        block_scatter<false>(a, sc, sp, ci, L);
     }
 }
 
@@ -177,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_g2p(AdjArgs q) {
 // and receives (dJ/dm, dJ/dp).
 __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ scenes, GridDev g, float4* gadj,
                                                   const uint32_t* __restrict__ dlist,
-                                                  const uint32_t* __restrict__ dcount, float dt) {
+                                                  const uint32_t* __restrict__ dcount, float dt, int use_esc) {
     const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
@@ -186,7 +430,7 @@ __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ s
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (t << 6) | uint32_t(lane);
-        float4 q = gadj[c];
+        float4 q = use_esc ? gadj[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
         const float4 mp = g.gmp[c];
         const float m = mp.x;
@@ -213,15 +457,12 @@ __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ s
 }
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_adj_p2g(AdjArgs q) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_p2g(AdjArgs q) {
     __shared__ double red[2][WAVES];
     const StepArgs& a = q.a;
     const uint32_t cap = a.cap;
-    const float* __restrict__ ps = a.ps;
     float* __restrict__ adj = q.adj;
-    const float dt = a.dt;
     const uint32_t nch = *a.n_chunks_dev;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
         const uint32_t ci = w.ci;
         const Chunk ch = a.chunks[ci];
@@ -229,209 +470,72 @@ __global__ __launch_bounds__(BLOCK) void k_adj_p2g(AdjArgs q) {
         const uint32_t slot = ch.start + threadIdx.x;
         double amu = 0.0, alam = 0.0;
         if (threadIdx.x < ch.count) {
-            float x[3], v[3], C[9], F[9];
+            float gx[3], gv[3], gC[9], gF[9];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) gx[k] = adj[(size_t)(AX + k) * cap + slot];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) gF[k] = adj[(size_t)(AF + k) * cap + slot];
+            float gE = adj[(size_t)AE * cap + slot];
+            adj_p2g_particle(a, q.gadj, sc, slot, gx, gF, gv, gC, gE, amu, alam);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                x[k] = ps[(size_t)(CX + k) * cap + slot];
-                v[k] = ps[(size_t)(CV + k) * cap + slot];
+                adj[(size_t)(AX + k) * cap + slot] = gx[k];
+                adj[(size_t)(AV + k) * cap + slot] = gv[k];
             }
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                C[k] = ps[(size_t)(CC + k) * cap + slot];
-                F[k] = ps[(size_t)(CF + k) * cap + slot];
+                adj[(size_t)(AC + k) * cap + slot] = gC[k];
+                adj[(size_t)(AF + k) * cap + slot] = gF[k];
             }
-            const float m = ps[(size_t)CMASS * cap + slot];
-            const float vol = ps[(size_t)CVOL * cap + slot];
-            float mu = sc.mu, lam = sc.lam;
-            if (a.has_E) {
-                const float E = ps[(size_t)CE * cap + slot];
-                mu = E / (2.0f * (1.0f + sc.poisson));
-                lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
-            }
-            // corotated-SVK PK1 from the FP64 SVD (mpm_cpu_diff.rs:279-287)
-            double U[9], s[3], V[9];
-            svd3_f64_raw(F, U, s, V);
-            svd3_signs<MODEL_CPUDIFF>(U, s, V);
-            const double dmu = mu, dlam = lam;
-            const double trsi = s[0] + s[1] + s[2] - 3.0;
-            Dm3 P;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double r = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
-                    const double fr = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
-                                      U[3 * i + 2] * s[2] * V[3 * j + 2];
-                    P(i, j) = 2.0 * dmu * (fr - r) + dlam * trsi * r;
-                }
-            // A = m C + 4/dx^2 stress, stress = -dt vol P F^T
-            const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
-            const double coef = -(double)dt * (double)vol;
-            float A[9];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double st = coef * (P(i, 0) * F[3 * j] + P(i, 1) * F[3 * j + 1] + P(i, 2) * F[3 * j + 2]);
-                    A[3 * i + j] = m * C[3 * i + j] + s4 * (float)st;
-                }
-            float mv[3] = {m * v[0], m * v[1], m * v[2]};
-            float xp[3], fx[3], w[3][3], dw[3][3];
-            int b[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
-                const float bf = floorf(xp[k] - 0.5f);
-                b[k] = int(bf);
-                fx[k] = xp[k] - bf;
-                quad_weights(fx[k], w[k]);
-                dw[k][0] = -(1.5f - fx[k]);
-                dw[k][1] = -2.0f * (fx[k] - 1.0f);
-                dw[k][2] = fx[k] - 0.5f;
-            }
-            // P2G reverse (mpm_adjoint.rs:389-487): with gw = sum w dJ/dp_i and
-            // T = sum w dJ/dp_i d^T:  dJ/dv = m gw, dJ/dC = m T,
-            // dJ/dstress = 4/dx^2 T, dJ/dx = sum grad(w) (dJ/dp_i . (m v + A d)
-            // + m dJ/dm_i) - A^T gw
-            const float dx = sc.dx, idx = sc.inv_dx;
-            float gw[3] = {0.f, 0.f, 0.f}, T[9], gpos[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < 9; ++k) T[k] = 0.f;
-            for (int k = 0; k < 3; ++k)
-                for (int j = 0; j < 3; ++j)
-#pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
-                        if (!in_grid(sc, ix, iy, iz)) continue;
-                        const float4 g = q.gadj[cell_index(sc, ix, iy, iz)];
-                        const float wt = w[0][i] * w[1][j] * w[2][k];
-                        const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
-                                    d2 = (float(k) - fx[2]) * dx;
-                        gw[0] += wt * g.y;
-                        gw[1] += wt * g.z;
-                        gw[2] += wt * g.w;
-                        const float ax = wt * g.y, ay = wt * g.z, az = wt * g.w;
-                        T[0] += ax * d0; T[1] += ax * d1; T[2] += ax * d2;
-                        T[3] += ay * d0; T[4] += ay * d1; T[5] += ay * d2;
-                        T[6] += az * d0; T[7] += az * d1; T[8] += az * d2;
-                        const float h = g.y * (mv[0] + A[0] * d0 + A[1] * d1 + A[2] * d2) +
-                                        g.z * (mv[1] + A[3] * d0 + A[4] * d1 + A[5] * d2) +
-                                        g.w * (mv[2] + A[6] * d0 + A[7] * d1 + A[8] * d2) + m * g.x;
-                        gpos[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
-                        gpos[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
-                        gpos[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
-                    }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                gpos[k] -= A[k] * gw[0] + A[3 + k] * gw[1] + A[6 + k] * gw[2];
-                adj[(size_t)(AX + k) * cap + slot] += gpos[k];
-                adj[(size_t)(AV + k) * cap + slot] = m * gw[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 9; ++k) adj[(size_t)(AC + k) * cap + slot] = m * T[k];
-            // stress = coef P F^T with dJ/dstress = s4 T:
-            //   dJ/dP = coef s4 T F,  dJ/dF += coef s4 T^T P
-            Dm3 aP, gf;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    aP(i, j) = coef * (double)s4 *
-                               ((double)T[3 * i] * F[j] + (double)T[3 * i + 1] * F[3 + j] + (double)T[3 * i + 2] * F[6 + j]);
-                    gf(i, j) = (double)adj[(size_t)(AF + 3 * i + j) * cap + slot] +
-                               coef * (double)s4 *
-                                   ((double)T[i] * P(0, j) + (double)T[3 + i] * P(1, j) + (double)T[6 + i] * P(2, j));
-                }
-            // corotated_pk1_adjoint (mpm_cpu_diff.rs:330-379), in the rotated
-            // frame: with Ut aP V = Q, aU = aP V (2mu Sd + cr I) - ..., the SVD
-            // adjoint only needs Ut aU, Vt aV, so form them directly.
-            Dm3 Q;  // U^T aP V
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const double pv = aP(r, 0) * V[j] + aP(r, 1) * V[3 + j] + aP(r, 2) * V[6 + j];
-                        acc += U[3 * r + i] * pv;
-                    }
-                    Q(i, j) = acc;
-                }
-            const double cr = -2.0 * dmu + dlam * trsi;
-            const double r_dot = Q(0, 0) + Q(1, 1) + Q(2, 2);  // <R, aP> = tr(U^T aP V)
-            // aU = aP V (2mu Sd) + cr aP V  ->  Ut aU = Q (2mu Sd + cr I)
-            // aV = aP^T U (2mu Sd) + cr aP^T U  ->  Vt aV = Q^T (2mu Sd + cr I)
-            Dm3 UaU, VaV;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double dj = 2.0 * dmu * s[j] + cr;
-                    UaU(i, j) = Q(i, j) * dj;
-                    VaV(i, j) = Q(j, i) * dj;
-                }
-            double aS[3];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) aS[i] = 2.0 * dmu * Q(i, i) + dlam * r_dot;
-            // <Fr - R, aP> = sum_i (s_i - 1) Q_ii
-            amu = 2.0 * ((s[0] - 1.0) * Q(0, 0) + (s[1] - 1.0) * Q(1, 1) + (s[2] - 1.0) * Q(2, 2));
-            alam = trsi * r_dot;
-            // adj_svd3 (dense3.rs:321-372): dJ/dF += U M V^T,
-            //   M = diag(aS) + (f o om_u) Sd + Sd (f o om_v),
-            //   om_u = Ut aU - aU^T U = UaU - UaU^T, om_v = VaV - VaV^T
-            const double f01 = safe_inv_sq_diff(s[0], s[1]);
-            const double f02 = safe_inv_sq_diff(s[0], s[2]);
-            const double f12 = safe_inv_sq_diff(s[1], s[2]);
-            Dm3 Mx;
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    double f = 0.0;
-                    if (i == 0 && j == 1) f = f01;
-                    if (i == 0 && j == 2) f = f02;
-                    if (i == 1 && j == 2) f = f12;
-                    if (i == 1 && j == 0) f = -f01;
-                    if (i == 2 && j == 0) f = -f02;
-                    if (i == 2 && j == 1) f = -f12;
-                    const double omu = UaU(i, j) - UaU(j, i);
-                    const double omv = VaV(i, j) - VaV(j, i);
-                    Mx(i, j) = (i == j ? aS[i] : 0.0) + f * omu * s[j] + s[i] * f * omv;
-                }
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    double acc = gf(i, j);
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-                        acc += U[3 * i + r] * (Mx(r, 0) * V[3 * j] + Mx(r, 1) * V[3 * j + 1] + Mx(r, 2) * V[3 * j + 2]);
-                    adj[(size_t)(AF + 3 * i + j) * cap + slot] = (float)acc;
-                }
-            // dJ/dE_p (per-particle E, chain rule through lame, mpm_cpu_diff.rs:217-222)
-            const double nu = sc.poisson;
-            const double ge = amu / (2.0 * (1.0 + nu)) + alam * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-            adj[(size_t)AE * cap + slot] += (float)ge;
+            adj[(size_t)AE * cap + slot] = gE;
         }
-        // per-chunk (mu, lambda) sums: one block owns the chunk, fixed order
-        amu = warp_sum(amu);
-        alam = warp_sum(alam);
-        if (lane == 0) {
-            red[0][wave] = amu;
-            red[1][wave] = alam;
+        add_chunk_mu_lambda(q.gml, ci, amu, alam, red);
+    }
+}
+
+// Reverse P2G of substep k fused with the reverse G2P of substep k-1 for
+// the same particle: dJ/d(state k) stays in registers.  Valid when no
+// particle of either substep scatters with global atomics (the host checks
+// the recompute's escape and fallback counters), so gadj is only read here.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_fused(AdjArgs q, const float* ps_prev, const float4* gv_prev) {
+    __shared__ ScatterLds L;
+    __shared__ double red[2][WAVES];
+    const StepArgs& a = q.a;
+    StepArgs ap = q.a;
+    ap.ps = ap.ps_out = const_cast<float*>(ps_prev);
+    ap.gv = const_cast<float4*>(gv_prev);
+    const uint32_t cap = a.cap;
+    float* __restrict__ adj = q.adj;
+    const uint32_t nch = *a.n_chunks_dev;
+    for (ChunkWalk w = chunk_walk(nch); w.ci < w.end; w.ci += w.step) {
+        const uint32_t ci = w.ci;
+        const Chunk ch = a.chunks[ci];
+        const SceneDev sc = a.scenes[ch.scene];
+        const uint32_t slot = ch.start + threadIdx.x;
+        const bool active = threadIdx.x < ch.count;
+        double amu = 0.0, alam = 0.0;
+        float gx[3], gv[3], gC[9], gF[9];
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) gx[k] = adj[(size_t)(AX + k) * cap + slot];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) gF[k] = adj[(size_t)(AF + k) * cap + slot];
+            float gE = adj[(size_t)AE * cap + slot];
+            adj_p2g_particle(a, q.gadj, sc, slot, gx, gF, gv, gC, gE, amu, alam);
+            adj[(size_t)AE * cap + slot] = gE;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double sm = 0.0, sl = 0.0;
-            for (int w = 0; w < WAVES; ++w) {
-                sm += red[0][w];
-                sl += red[1][w];
-            }
-            q.gml[2 * (size_t)ci] += sm;
-            q.gml[2 * (size_t)ci + 1] += sl;
+        add_chunk_mu_lambda(q.gml, ci, amu, alam, red);
+        Scatter sp;
+        sp.on = false;
+        sp.esc = false;
+        if (active) {
+            adj_g2p_particle(ap, sc, slot, gx, gv, gC, gF, sp);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) adj[(size_t)(AX + k) * cap + slot] = gx[k];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) adj[(size_t)(AF + k) * cap + slot] = gF[k];
         }
-        __syncthreads();
+        block_scatter<false>(ap, sc, sp, ci, L);
     }
 }
 
@@ -443,8 +547,16 @@ hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s) {
 }
 
 hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
-                           const uint32_t* dcount, float dt, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_grid, dim3(blocks), dim3(256), 0, s, scenes, g, gadj, dlist, dcount, dt);
+                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_grid, dim3(blocks), dim3(256), 0, s, scenes, g, gadj, dlist, dcount, dt, use_esc);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_fused(const AdjArgs& q, const float* ps_prev, const float4* gv_prev, int blocks,
+                            hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);
+    hipLaunchKernelGGL(k_adj_fused, dim3(blocks), dim3(BLOCK), 0, s, q, ps_prev, gv_prev);
     return hipGetLastError();
 }
 

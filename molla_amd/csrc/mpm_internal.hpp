@@ -190,11 +190,17 @@ struct AdjArgs {
 hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s);
 // Grid reverse over the active tiles: dJ/d(grid v) -> (dJ/dm, dJ/dp) in gadj;
 // g.gmp holds the substep's P2G total (m, p).
+// use_esc = 0: gadj holds no escaped-particle part (it is overwritten).
 hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
-                           const uint32_t* dcount, float dt, int blocks, hipStream_t s);
+                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s);
 // Reverse P2G + stress: gathers gadj; writes adj x, v, C, F of the pre-state
 // and accumulates adj E and the per-chunk (mu, lambda) sums.
 hipError_t launch_adj_p2g(const AdjArgs& q, int blocks, hipStream_t s);
+// Reverse P2G of the substep of q.a.ps fused with the reverse G2P of the
+// previous substep (pre-state ps_prev, grid velocity gv_prev).  Only valid
+// when neither scatters with global atomics.
+hipError_t launch_adj_fused(const AdjArgs& q, const float* ps_prev, const float4* gv_prev, int blocks,
+                            hipStream_t s);
 
 // ---- scans and the sorted chunk list (mpm_scan.hip) ----
 size_t scan_scratch_words(size_t n);

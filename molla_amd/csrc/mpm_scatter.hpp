@@ -155,7 +155,9 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
 // bounding box of the block's stencils; the box is stored as the chunk's
 // partial with plain stores (no global atomics, no registration: the grid
 // update finds the chunk through the sort).  Escaped particles, and blocks
-// whose box exceeds LDS_CELLS, scatter with global atomics.
+// whose box exceeds LDS_CELLS, scatter with global atomics.  MASS = false
+// skips the mass channel (left 0 in the partials) for scatters whose m is 0.
+template <bool MASS = true>
 __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
                                               uint32_t chunk, ScatterLds& L) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -233,7 +235,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         const int vol = bx * by * L.box[5];
         const double sm = L.scale[0], sp = L.scale[1];
         for (int i = tid; i < vol; i += BLOCK) {
-            L.acc[0][i] = 0ull;
+            if (MASS) L.acc[0][i] = 0ull;
             L.acc[1][i] = 0ull;
             L.acc[2][i] = 0ull;
             L.acc[3][i] = 0ull;
@@ -254,7 +256,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                         float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
                               d2 = (float(k) - p.fx[2]) * dx;
                         int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-                        lds_add_u64(&L.acc[0][li], fixed_point(wt * p.m, sm));
+                        if (MASS) lds_add_u64(&L.acc[0][li], fixed_point(wt * p.m, sm));
                         lds_add_u64(&L.acc[1][li], fixed_point(wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2), sp));
                         lds_add_u64(&L.acc[2][li], fixed_point(wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2), sp));
                         lds_add_u64(&L.acc[3][li], fixed_point(wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2), sp));
@@ -264,7 +266,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         const double im = 1.0 / sm, ip = 1.0 / sp;
         float4* part = a.part + (size_t)chunk * LDS_CELLS;
         for (int i = tid; i < vol; i += BLOCK)
-            part[i] = make_float4((float)((double)(long long)L.acc[0][i] * im),
+            part[i] = make_float4(MASS ? (float)((double)(long long)L.acc[0][i] * im) : 0.f,
                                   (float)((double)(long long)L.acc[1][i] * ip),
                                   (float)((double)(long long)L.acc[2][i] * ip),
                                   (float)((double)(long long)L.acc[3][i] * ip));
