@@ -254,6 +254,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay step_n as a hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB (for counter scaling)")
@@ -314,6 +315,8 @@ def main():
             out["batched"] = {"error": str(e)}
 
     try:
+        if args.no_plain:
+            raise RuntimeError("skipped (--no-plain)")
         out["plain"] = run_plain(args.scale)
         log(f"plain: {out['plain']}")
         if "batched" in out and "value" in out["batched"]:
