@@ -380,7 +380,7 @@ __device__ __forceinline__ void add_chunk_mu_lambda(double* gml, uint32_t ci, do
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_adj_g2p(AdjArgs q) {
-    __shared__ ScatterLds L;
+    __shared__ ScatterLdsT<true> L;
     const StepArgs& a = q.a;
     const uint32_t cap = a.cap;
     float* __restrict__ adj = q.adj;
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
 #pragma unroll
             for (int k = 0; k < 9; ++k) adj[(size_t)(AF + k) * cap + slot] = gF[k];
         }
-        block_scatter<false>(a, sc, sp, ci, L);
+        block_scatter<false, true>(a, sc, sp, ci, L);
     }
 }
 
@@ -498,7 +498,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 // particle of either substep scatters with global atomics (the host checks
 // the recompute's escape and fallback counters), so gadj is only read here.
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_adj_fused(AdjArgs q, const float* ps_prev, const float4* gv_prev) {
-    __shared__ ScatterLds L;
+    __shared__ ScatterLdsT<true> L;
     __shared__ double red[2][WAVES];
     const StepArgs& a = q.a;
     StepArgs ap = q.a;
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 #pragma unroll
             for (int k = 0; k < 9; ++k) adj[(size_t)(AF + k) * cap + slot] = gF[k];
         }
-        block_scatter<false>(ap, sc, sp, ci, L);
+        block_scatter<false, true>(ap, sc, sp, ci, L);
     }
 }
 

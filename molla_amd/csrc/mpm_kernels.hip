@@ -61,7 +61,10 @@ __device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float*
 // the pass is latency-bound, and this beats the unconstrained 2 waves/SIMD.
 template <int MODE, int MODEL>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_particles(StepArgs a) {
-    __shared__ ScatterLds L;
+    // The differentiable model scatters in int64: its reverse pass divides by
+    // grid masses, so edge cells need each contribution kept to f32 precision.
+    constexpr bool WIDE = MODEL == MODEL_CPUDIFF;
+    __shared__ ScatterLdsT<WIDE> L;
     if (threadIdx.x == 0) L.box[15] = a.status[ST_FAULT] != 0u ? 1 : 0;
     __syncthreads();
     if (L.box[15]) return;  // block-uniform: a latched fault stops the pass
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 if (a.use_lds && stencil_escapes(a, sc, slot, sp.b)) sp.esc = true;
             }
         }
-        if (MODE != MODE_G2P) block_scatter(a, sc, sp, ci, L);
+        if (MODE != MODE_G2P) block_scatter<true, WIDE>(a, sc, sp, ci, L);
     }
 }
 

@@ -3,6 +3,7 @@
 // block-level P2G scatter (LDS-reduced per-chunk partials with a global
 // atomic fallback).
 #pragma once
+#include <type_traits>
 #include "mpm_device.hpp"
 #include "mpm_internal.hpp"
 
@@ -89,20 +90,39 @@ struct Scatter {
 };
 
 // LDS workspace of the block scatter.  The block's box is accumulated as
-// 64-bit fixed point: on gfx950 LDS integer atomics run at about the LDS
+// 32-bit fixed point: on gfx950 LDS integer atomics run at about the LDS
 // read rate while LDS f32 atomic adds are ~20x slower, and integer sums are
 // independent of the order in which lanes and waves arrive.
-struct ScatterLds {
-    unsigned long long acc[4][LDS_CELLS];  // (m, px, py, pz): 64 KiB
+//   narrow (int32): each contribution is rounded to 2^-23 of the power of
+//     two above the block's bound on its values -- about the precision of f32
+//     accumulation for the forward P2G, whose contributions in a block are of
+//     one magnitude;
+//   wide (int64): each f32 contribution converts exactly (2^-50 of the
+//     largest), for sums whose terms span many magnitudes and cancel (the
+//     adjoint's dJ/d grid v).
+template <bool WIDE>
+struct ScatterLdsT {
+    typename std::conditional<WIDE, unsigned long long, int>::type acc[4][LDS_CELLS];  // (m, px, py, pz)
     int red[WAVES * 6];
     float redf[WAVES * 2];
     int box[16];
-    double scale[2];  // fixed-point scale of the mass and momentum channels
+    double scale[2];  // power-of-two fixed-point scale of the mass and momentum channels
 };
+using ScatterLds = ScatterLdsT<false>;
+
+// round(a * b) as an int32 for |a * b| < 2^22: one f32 FMA with 1.5 * 2^23
+// rounds the product once and leaves the integer in the low mantissa bits.
+__device__ __forceinline__ int fixed32(float a, float b) {
+    return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
+}
+
+__device__ __forceinline__ void lds_add_i32(int* p, int v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 
 // round(v * s) as a two's-complement int64 for |v * s| < 2^51: adding
 // 1.5 * 2^52 leaves the rounded integer in the low mantissa bits.
-__device__ __forceinline__ unsigned long long fixed_point(float v, double s) {
+__device__ __forceinline__ unsigned long long fixed64(float v, double s) {
     const double t = (double)v * s + 6755399441055744.0;
     return (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000ll);
 }
@@ -157,9 +177,9 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
 // update finds the chunk through the sort).  Escaped particles, and blocks
 // whose box exceeds LDS_CELLS, scatter with global atomics.  MASS = false
 // skips the mass channel (left 0 in the partials) for scatters whose m is 0.
-template <bool MASS = true>
+template <bool MASS = true, bool WIDE = false>
 __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
-                                              uint32_t chunk, ScatterLds& L) {
+                                              uint32_t chunk, ScatterLdsT<WIDE>& L) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float dx = sc.dx;
     const bool lds_on = p.on && !p.esc;
@@ -207,9 +227,14 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             mm = fmaxf(mm, L.redf[w * 2]);
             mp = fmaxf(mp, L.redf[w * 2 + 1]);
         }
-        // contributions stay below 2^50 in fixed point; 256 of them sum below 2^58
-        L.scale[0] = mm > 0.f && mm == mm ? ldexp(1.0, 50 - exp2_ceil(mm)) : 1.0;
-        L.scale[1] = mp > 0.f && mp == mp ? ldexp(1.0, 50 - exp2_ceil(mp)) : 1.0;
+        // A quadratic B-spline weight product is at most 0.75^3 < 0.422, so
+        // with values scaled below 2^23 every contribution w q stays below
+        // 0.422 * 2^23 < 2^22 (fixed32's range) and 256 of them below
+        // 0.85 * 2^31 (no int32 overflow).  Wide: values below 2^50, sums of
+        // 256 below 2^58.
+        const int top = WIDE ? 50 : 23;
+        L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, top - exp2_ceil(mm)) : 1.0;
+        L.scale[1] = mp > 0.f && mp < 1e30f ? ldexp(1.0, top - exp2_ceil(mp)) : 1.0;
         const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
         int use = 2;  // nothing reduced in LDS
         if (bx > 0 && by > 0 && bz > 0)
@@ -233,12 +258,15 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
     if (use == 1) {
         const int x0 = L.box[0], y0 = L.box[1], z0 = L.box[2], bx = L.box[3], by = L.box[4];
         const int vol = bx * by * L.box[5];
-        const double sm = L.scale[0], sp = L.scale[1];
+        const double sm64 = L.scale[0], sp64 = L.scale[1];
+        // narrow: node values are pre-scaled and rounded by fixed32; wide:
+        // node values stay unscaled and fixed64 scales them exactly
+        const float sm = WIDE ? 1.0f : (float)sm64, sp = WIDE ? 1.0f : (float)sp64;
         for (int i = tid; i < vol; i += BLOCK) {
-            if (MASS) L.acc[0][i] = 0ull;
-            L.acc[1][i] = 0ull;
-            L.acc[2][i] = 0ull;
-            L.acc[3][i] = 0ull;
+            if (MASS) L.acc[0][i] = 0;
+            L.acc[1][i] = 0;
+            L.acc[2][i] = 0;
+            L.acc[3][i] = 0;
         }
         __syncthreads();
         if (lds_on) {
@@ -246,30 +274,66 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             quad_weights(p.fx[0], w[0]);
             quad_weights(p.fx[1], w[1]);
             quad_weights(p.fx[2], w[2]);
-            for (int k = 0; k < 3; ++k)
-                for (int j = 0; j < 3; ++j)
+            // With d = ((i, j, k) - fx) dx the momentum term is linear in the
+            // node offset: mv + A d = c + i a0 + j a1 + k a2, c = mv - dx A fx,
+            // a_q = dx A[:, q]; all pre-scaled to fixed point.
+            float a0[3], a1[3], a2[3], c[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                a0[r] = dx * p.A[3 * r] * sp;
+                a1[r] = dx * p.A[3 * r + 1] * sp;
+                a2[r] = dx * p.A[3 * r + 2] * sp;
+                c[r] = p.mv[r] * sp - (a0[r] * p.fx[0] + a1[r] * p.fx[1] + a2[r] * p.fx[2]);
+            }
+            const float ms = p.m * sm;
+            const int G = int(sc.G);
+            const int li0 = (p.b[0] - x0) + bx * ((p.b[1] - y0) + by * (p.b[2] - z0));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int iz = p.b[2] + k;
+                if (iz < 0 || iz >= G) continue;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int iy = p.b[1] + j;
+                    if (iy < 0 || iy >= G) continue;
+                    const float wjk = w[1][j] * w[2][k];
+                    const int lrow = li0 + bx * (j + by * k);
+                    float q0 = c[0] + float(j) * a1[0] + float(k) * a2[0];
+                    float q1 = c[1] + float(j) * a1[1] + float(k) * a2[1];
+                    float q2 = c[2] + float(j) * a1[2] + float(k) * a2[2];
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
-                        int ix = p.b[0] + i, iy = p.b[1] + j, iz = p.b[2] + k;
-                        if (!in_grid(sc, ix, iy, iz)) continue;
-                        float wt = w[0][i] * w[1][j] * w[2][k];
-                        float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
-                              d2 = (float(k) - p.fx[2]) * dx;
-                        int li = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-                        if (MASS) lds_add_u64(&L.acc[0][li], fixed_point(wt * p.m, sm));
-                        lds_add_u64(&L.acc[1][li], fixed_point(wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2), sp));
-                        lds_add_u64(&L.acc[2][li], fixed_point(wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2), sp));
-                        lds_add_u64(&L.acc[3][li], fixed_point(wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2), sp));
+                        const int ix = p.b[0] + i;
+                        if (ix >= 0 && ix < G) {
+                            const float wt = w[0][i] * wjk;
+                            if constexpr (WIDE) {
+                                if (MASS) lds_add_u64(&L.acc[0][lrow + i], fixed64(wt * ms, sm64));
+                                lds_add_u64(&L.acc[1][lrow + i], fixed64(wt * q0, sp64));
+                                lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
+                                lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
+                            } else {
+                                if (MASS) lds_add_i32(&L.acc[0][lrow + i], fixed32(wt, ms));
+                                lds_add_i32(&L.acc[1][lrow + i], fixed32(wt, q0));
+                                lds_add_i32(&L.acc[2][lrow + i], fixed32(wt, q1));
+                                lds_add_i32(&L.acc[3][lrow + i], fixed32(wt, q2));
+                            }
+                        }
+                        q0 += a0[0];
+                        q1 += a0[1];
+                        q2 += a0[2];
                     }
+                }
+            }
         }
         __syncthreads();
-        const double im = 1.0 / sm, ip = 1.0 / sp;
+        const double im = 1.0 / sm64, ip = 1.0 / sp64;
         float4* part = a.part + (size_t)chunk * LDS_CELLS;
-        for (int i = tid; i < vol; i += BLOCK)
-            part[i] = make_float4(MASS ? (float)((double)(long long)L.acc[0][i] * im) : 0.f,
-                                  (float)((double)(long long)L.acc[1][i] * ip),
-                                  (float)((double)(long long)L.acc[2][i] * ip),
-                                  (float)((double)(long long)L.acc[3][i] * ip));
+        for (int i = tid; i < vol; i += BLOCK) {
+            using acc_t = typename std::conditional<WIDE, long long, int>::type;
+            part[i] = make_float4(MASS ? (float)((double)(acc_t)L.acc[0][i] * im) : 0.f,
+                                  (float)((double)(acc_t)L.acc[1][i] * ip), (float)((double)(acc_t)L.acc[2][i] * ip),
+                                  (float)((double)(acc_t)L.acc[3][i] * ip));
+        }
         __syncthreads();
     }
     if (p.on && (p.esc || use == 0)) {

@@ -66,7 +66,12 @@ def test_grid_mass_matches_particle_mass_and_oracle_p2g():
     assert abs(m.astype(np.float64).sum() - exp) / exp < 1e-4
     assert np.abs(p).max() < 1e-6
     gm, gp = orc.p2g(cfg, oracle_state(cfg, x), 1 / 240)
-    assert rel_err(m, gm) < 1e-6
+    # cellwise against FP64, bounded by 4x what a sequential f32 scatter (the
+    # reference's accumulation) reaches on the same input
+    m32, _ = orc.step(cfg, oracle_state(cfg, x, dtype=np.float32), 1 / 240, 1, want_grid=True)
+    e, e32 = rel_err(m, gm), rel_err(m32, gm)
+    log(f"  cell mass err {e:.2e} (f32 scatter {e32:.2e})")
+    assert e < max(4 * e32, 1e-6)
 
 
 def test_p2g_with_stress_matches_oracle():
@@ -89,9 +94,13 @@ def test_p2g_with_stress_matches_oracle():
         st = oracle_state(cfg, x, v.astype(np.float32), F=F.astype(np.float32))
         st.C[:] = C.astype(np.float32)
         gm, gp = orc.p2g(cfg, st, 1e-4)
+        s32 = oracle_state(cfg, x, v.astype(np.float32), F=F.astype(np.float32), dtype=np.float32)
+        s32.C[:] = C.astype(np.float32)
+        m32, p32 = orc.step(cfg, s32, 1e-4, 1, want_grid=True)
         em, ep = rel_err(m, gm), rel_err(p, gp)
-        log(f"  mass err {em:.2e} momentum err {ep:.2e}")
-        assert em < 1e-6 and ep < 1e-5
+        em32, ep32 = rel_err(m32, gm), rel_err(p32, gp)
+        log(f"  mass err {em:.2e} momentum err {ep:.2e} (f32 scatter {em32:.2e} {ep32:.2e})")
+        assert em < max(4 * em32, 1e-6) and ep < max(4 * ep32, 1e-6)
 
 
 # ---------------------------------------------------------------- whole steps
