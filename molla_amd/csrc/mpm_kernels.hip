@@ -147,30 +147,55 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                         fx[k] = xp[k] - bf;
                         quad_weights(fx[k], w[k]);
                     }
+                    // v = sum w v_i and, with d = ((i, j, k) - fx) dx,
+                    // C = 4/dx^2 sum w v_i d^T = 4/dx (M - v fx^T),
+                    // M = sum w v_i (i, j, k)^T, accumulated per x-row.
                     float vn[3] = {0.f, 0.f, 0.f};
-                    float Cn[9];
+                    float M[9];
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) Cn[k] = 0.f;
-                    const float dx = sc.dx, s4 = 4.0f * sc.inv_dx * sc.inv_dx;
-                    for (int k = 0; k < 3; ++k)
-                        for (int j = 0; j < 3; ++j)
+                    for (int k = 0; k < 9; ++k) M[k] = 0.f;
+                    const int G = int(sc.G);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int iz = b[2] + k;
+                        if (iz < 0 || iz >= G) continue;
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) {
+                            const int iy = b[1] + j;
+                            if (iy < 0 || iy >= G) continue;
+                            const float wjk = w[1][j] * w[2][k];
+                            float rs[3] = {0.f, 0.f, 0.f}, ri[3] = {0.f, 0.f, 0.f};
 #pragma unroll
                             for (int i = 0; i < 3; ++i) {
-                                int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
-                                if (!in_grid(sc, ix, iy, iz)) continue;
-                                float4 g = a.gv[cell_index(sc, ix, iy, iz)];
-                                float wt = w[0][i] * w[1][j] * w[2][k];
-                                float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
-                                      d2 = (float(k) - fx[2]) * dx;
-                                vn[0] += wt * g.x;
-                                vn[1] += wt * g.y;
-                                vn[2] += wt * g.z;
-                                float sw = s4 * wt;
-                                float gx = sw * g.x, gy = sw * g.y, gz = sw * g.z;
-                                Cn[0] += gx * d0; Cn[1] += gx * d1; Cn[2] += gx * d2;
-                                Cn[3] += gy * d0; Cn[4] += gy * d1; Cn[5] += gy * d2;
-                                Cn[6] += gz * d0; Cn[7] += gz * d1; Cn[8] += gz * d2;
+                                const int ix = b[0] + i;
+                                if (ix < 0 || ix >= G) continue;
+                                const float4 g = a.gv[cell_index(sc, ix, iy, iz)];
+                                const float wt = w[0][i] * wjk;
+                                const float gx = wt * g.x, gy = wt * g.y, gz = wt * g.z;
+                                rs[0] += gx;
+                                rs[1] += gy;
+                                rs[2] += gz;
+                                if (i > 0) {
+                                    ri[0] += float(i) * gx;
+                                    ri[1] += float(i) * gy;
+                                    ri[2] += float(i) * gz;
+                                }
                             }
+#pragma unroll
+                            for (int r = 0; r < 3; ++r) {
+                                vn[r] += rs[r];
+                                M[3 * r] += ri[r];
+                                M[3 * r + 1] += float(j) * rs[r];
+                                M[3 * r + 2] += float(k) * rs[r];
+                            }
+                        }
+                    }
+                    const float dx = sc.dx, s4dx = 4.0f * sc.inv_dx;
+                    float Cn[9];
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) Cn[3 * r + c] = s4dx * (M[3 * r + c] - vn[r] * fx[c]);
                     const int rheo = int(mrow[0] + 0.5f);
                     if (MODEL == MODEL_SHADER && rheo == RHEO_VISCOUS) {
                         float vd = __expf(-fmaxf(mrow[5], 0.0f) * dt);
