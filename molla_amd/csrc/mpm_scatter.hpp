@@ -366,7 +366,7 @@ __device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDe
 //   2. the deduplicated ranges are laid out as one candidate list, one
 //      candidate per lane, whose boxes load in parallel,
 //   3. a ballot keeps the boxes overlapping the tile, and the wave adds
-//      their partials in ascending order, two loads in flight at a time.
+//      their partials in ascending order, eight loads in flight at a time.
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
                                              int ix, int iy, int iz, float4& q) {
     const int lane = threadIdx.x & 63;
@@ -415,41 +415,39 @@ __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& s
         }
         uint64_t m = __ballot(hit);
         while (m) {
-            const int l0 = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1;
-            const bool two = m != 0;
-            const int l1 = two ? __ffsll((unsigned long long)m) - 1 : l0;
-            if (two) m &= m - 1;
-            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
-            {
-                const int ox = __builtin_amdgcn_readlane(o.x, l0), oy = __builtin_amdgcn_readlane(o.y, l0),
-                          oz = __builtin_amdgcn_readlane(o.z, l0);
-                const int bx = __builtin_amdgcn_readlane(d.x, l0), by = __builtin_amdgcn_readlane(d.y, l0),
-                          bz = __builtin_amdgcn_readlane(d.z, l0);
-                const int cid = __builtin_amdgcn_readlane(mine, l0);
-                const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
-                if (lx >= 0 && ly >= 0 && lz >= 0 && lx < bx && ly < by && lz < bz)
-                    r0 = g.part[(size_t)cid * LDS_CELLS + lx + bx * (ly + by * lz)];
+            // up to eight overlapping chunks per round: their loads are all
+            // in flight before the (ascending-order) adds
+            constexpr int U = 8;
+            int ls[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ls[u] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                m &= m - 1;
             }
-            if (two) {
-                const int ox = __builtin_amdgcn_readlane(o.x, l1), oy = __builtin_amdgcn_readlane(o.y, l1),
-                          oz = __builtin_amdgcn_readlane(o.z, l1);
-                const int bx = __builtin_amdgcn_readlane(d.x, l1), by = __builtin_amdgcn_readlane(d.y, l1),
-                          bz = __builtin_amdgcn_readlane(d.z, l1);
-                const int cid = __builtin_amdgcn_readlane(mine, l1);
-                const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
-                if (lx >= 0 && ly >= 0 && lz >= 0 && lx < bx && ly < by && lz < bz)
-                    r1 = g.part[(size_t)cid * LDS_CELLS + lx + bx * (ly + by * lz)];
+            float4 r[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                r[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (ls[u] >= 0) {
+                    const int l = ls[u];
+                    const int ox = __builtin_amdgcn_readlane(o.x, l), oy = __builtin_amdgcn_readlane(o.y, l),
+                              oz = __builtin_amdgcn_readlane(o.z, l);
+                    const int bx = __builtin_amdgcn_readlane(d.x, l), by = __builtin_amdgcn_readlane(d.y, l),
+                              bz = __builtin_amdgcn_readlane(d.z, l);
+                    const int cid = __builtin_amdgcn_readlane(mine, l);
+                    const int lx = ix - ox, ly = iy - oy, lz = iz - oz;
+                    if (lx >= 0 && ly >= 0 && lz >= 0 && lx < bx && ly < by && lz < bz)
+                        r[u] = g.part[(size_t)cid * LDS_CELLS + lx + bx * (ly + by * lz)];
+                }
             }
-            q.x += r0.x;
-            q.y += r0.y;
-            q.z += r0.z;
-            q.w += r0.w;
-            if (two) {
-                q.x += r1.x;
-                q.y += r1.y;
-                q.z += r1.z;
-                q.w += r1.w;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (ls[u] >= 0) {
+                    q.x += r[u].x;
+                    q.y += r[u].y;
+                    q.z += r[u].z;
+                    q.w += r[u].w;
+                }
             }
         }
     }
