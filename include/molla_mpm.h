@@ -150,6 +150,10 @@ int mpm_set_p2g_mode(mpm_sim* sim, int mode);
 /* Re-sort particles by 4^3 tile every `every` substeps (0 = only when
  * stepping starts).  Sorting never changes results, only locality. */
 int mpm_set_sort_interval(mpm_sim* sim, uint32_t every);
+/* set_time_step (mpm.rs:1632): 0 = MpmTimeStep::Explicit, k > 0 =
+ * Implicit { jacobi_iters: k } -- k semi-implicit Jacobi sweeps of the grid
+ * velocity after each grid update (mpm.wgsl:1060-1153).  Shader model only. */
+int mpm_set_time_step(mpm_sim* sim, uint32_t jacobi_iters);
 /* Store v and C at every substep (1, default, reference semantics) or only
  * at the last substep of a step_n call (0). */
 int mpm_set_write_vc(mpm_sim* sim, int every_substep);

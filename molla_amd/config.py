@@ -110,7 +110,7 @@ class Explicit:
 
 @dataclass(frozen=True)
 class Implicit:
-    """Semi-implicit Jacobi step (mpm.rs:382-392); not implemented on the HIP path."""
+    """Semi-implicit Jacobi step (mpm.rs:382-392): ``jacobi_iters`` grid sweeps per substep."""
 
     jacobi_iters: int = 3
 

@@ -95,6 +95,8 @@ struct mpm_sim {
     int has_E = 0;
     // options
     int p2g_mode = MPM_P2G_FUSED;
+    uint32_t jacobi_iters = 0;  // semi-implicit sweeps per substep (MpmTimeStep::Implicit)
+    float4* gv2 = nullptr;      // their ping-pong buffer
     uint32_t sort_every = 16;
     int write_vc = 1;
     uint32_t pblocks = 0;
@@ -208,7 +210,7 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml};
+    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2};
     for (void* b : dbufs)
         if (b) hipFree(b);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -305,6 +307,7 @@ int ensure_grid(mpm_sim* s) {
     const size_t cells = (size_t)s->total_tiles * 64;
     HIP_TRY(s, dalloc(&s->gmp, cells));
     HIP_TRY(s, dalloc(&s->gv, cells));
+    if (s->jacobi_iters > 0 && !s->gv2) HIP_TRY(s, dalloc(&s->gv2, cells));
     HIP_TRY(s, dalloc(&s->tile_flag, s->total_tiles));
     HIP_TRY(s, dalloc(&s->tile_scene, s->total_tiles));
     HIP_TRY(s, dalloc(&s->list[0], s->total_tiles));
@@ -428,6 +431,21 @@ int grid_update(mpm_sim* s, float dt, bool zero_after) {
     if (s->dry) return MPM_OK;
     Stage st(s, MPM_STAGE_GRID);
     const int nxt = 1 - s->list_cur;
+    if (s->jacobi_iters > 0) {
+        // Semi-implicit step (mpm.rs:2515-2530): the sweeps need this substep's
+        // tile flags, so the grid update keeps them and the clear follows.
+        const mpm::GridDev g = griddev(s);
+        uint32_t* cur_count = s->status + mpm::ST_COUNT0 + s->list_cur;
+        HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count, nullptr, dt, 0,
+                                           s->p2g_mode == MPM_P2G_FUSED ? 1 : 0, grid_blocks(s), s->stream));
+        HIP_TRY(s, mpm::launch_implicit_sweeps(s->d_scenes, g, s->gv2, s->jacobi_iters, s->list[s->list_cur],
+                                               cur_count, dt, grid_blocks(s), s->stream));
+        if (zero_after) {
+            HIP_TRY(s, mpm::launch_clear_tiles(g, s->list[s->list_cur], cur_count, grid_blocks(s), s->stream));
+            HIP_TRY(s, mpm::launch_set_u32(s->status + mpm::ST_COUNT0 + nxt, 0u, s->stream));
+        }
+        return MPM_OK;
+    }
     HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, griddev(s), s->list[s->list_cur],
                                        s->status + mpm::ST_COUNT0 + s->list_cur,
                                        zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, dt,
@@ -1123,6 +1141,22 @@ int mpm_set_p2g_mode(mpm_sim* sim, int mode) {
     if (mode == MPM_P2G_DETERMINISTIC && sim->sh[0].model != MPM_MODEL_SHADER)
         return fail(sim, MPM_ERR_INVALID, "deterministic P2G is defined for the shader model");
     sim->p2g_mode = mode;
+    return MPM_OK;
+}
+
+int mpm_set_time_step(mpm_sim* sim, uint32_t jacobi_iters) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (jacobi_iters > 0)
+        for (uint32_t i = 0; i < sim->n_scenes; ++i)
+            if (sim->cfg[i].model != MPM_MODEL_SHADER)
+                return fail(sim, MPM_ERR_INVALID, "the semi-implicit step is defined for MPM_MODEL_SHADER scenes");
+    if (jacobi_iters > 1024) return fail(sim, MPM_ERR_INVALID, "jacobi_iters must be <= 1024");
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    if (jacobi_iters > 0 && sim->gmp && !sim->gv2)
+        HIP_TRY(sim, dalloc(&sim->gv2, (size_t)sim->total_tiles * 64));
+    sim->jacobi_iters = jacobi_iters;
+    for (auto& kv : sim->graphs) hipGraphExecDestroy(kv.second);
+    sim->graphs.clear();
     return MPM_OK;
 }
 

@@ -145,6 +145,13 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
                               float4* keep = nullptr);
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also = nullptr);
+// `iters` semi-implicit Jacobi sweeps of g.gv over the active tiles (scratch:
+// a second cell buffer); the result is left in g.gv.  Tile flags must still
+// mark this substep's dynamic tiles (FLAG_DYN).
+hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, float4* scratch, uint32_t iters,
+                                  const uint32_t* dlist, const uint32_t* dcount, float dt, int blocks,
+                                  hipStream_t s);
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 // Static active list of a sort: the 3x3x3 dilation of occupied tiles, in
 // ascending tile order (mark/pos: total_tiles u32 each; scan_scratch:
 // scan_scratch_words(total_tiles)).  Also builds the tile lower-bound table.

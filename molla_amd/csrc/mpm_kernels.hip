@@ -482,6 +482,102 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
 
 // Zero the accumulation grid of the static and dynamic lists; keep only the
 // static bit.
+// ---------------------------------------------------------------------------
+// Semi-implicit Jacobi sweep (mpm_implicit_apply, mpm.wgsl:1060-1153) over the
+// active tiles, one wave per tile, one lane per cell, reading `src` and
+// writing `dst` (the host ping-pongs, mpm_implicit_swap):
+//   v' = (m v + a v_avg) / (m + a),  a = dt^2 m (E / 1500) / dx^2,
+// v_avg = mean velocity of the face neighbours with m > 1e-9 (v if none);
+// empty cells are copied.  A neighbour in a tile that is not on this
+// substep's lists (neither FLAG_STATIC nor FLAG_DYN) holds no P2G mass, as
+// in the reference's sparse grid, whatever its stale contents.
+__global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restrict__ scenes, GridDev g,
+                                                        const float4* __restrict__ src, float4* __restrict__ dst,
+                                                        const uint32_t* __restrict__ dlist,
+                                                        const uint32_t* __restrict__ dcount, float dt) {
+    const uint32_t ns = *g.scount, nd = *dcount;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+        const SceneDev sc = scenes[g.tile_scene[t]];
+        int tx, ty, tz, ix, iy, iz;
+        tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
+        const uint32_t c = (t << 6) | uint32_t(lane);
+        const float4 me = src[c];
+        const float m = me.w;
+        if (m <= 1.0e-9f || m != m) {
+            dst[c] = me;
+            continue;
+        }
+        const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+        float ax = 0.f, ay = 0.f, az = 0.f, cnt = 0.f;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int jx = ix + off[k][0], jy = iy + off[k][1], jz = iz + off[k][2];
+            if (!in_grid(sc, jx, jy, jz)) continue;
+            const uint32_t nc = cell_index(sc, jx, jy, jz);
+            if ((g.tile_flag[nc >> 6] & (FLAG_STATIC | FLAG_DYN)) == 0u) continue;
+            const float4 nb = src[nc];
+            if (nb.w <= 1.0e-9f || nb.w != nb.w) continue;
+            ax += nb.x;
+            ay += nb.y;
+            az += nb.z;
+            cnt += 1.0f;
+        }
+        if (cnt > 0.f) {
+            ax = ax / cnt;
+            ay = ay / cnt;
+            az = az / cnt;
+        } else {
+            ax = me.x;
+            ay = me.y;
+            az = me.z;
+        }
+        const float alpha = dt * dt * m * (sc.youngs / 1500.0f) * sc.inv_dx * sc.inv_dx;
+        const float den = m + alpha;
+        float vx = (m * me.x + alpha * ax) / den, vy = (m * me.y + alpha * ay) / den,
+              vz = (m * me.z + alpha * az) / den;
+        if (vx != vx || vy != vy || vz != vz) {
+            vx = me.x;
+            vy = me.y;
+            vz = me.z;
+        }
+        dst[c] = make_float4(vx, vy, vz, m);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy_tiles(GridDev g, const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    const uint32_t* __restrict__ dlist,
+                                                    const uint32_t* __restrict__ dcount) {
+    const uint32_t ns = *g.scount, nd = *dcount;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+        dst[(t << 6) | uint32_t(lane)] = src[(t << 6) | uint32_t(lane)];
+    }
+}
+
+hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, float4* scratch, uint32_t iters,
+                                  const uint32_t* dlist, const uint32_t* dcount, float dt, int blocks,
+                                  hipStream_t s) {
+    if (iters == 0) return hipSuccess;
+    // odd sweep counts start from a copy so the result lands in g.gv
+    float4* a = g.gv;
+    float4* b = scratch;
+    if (iters & 1u) {
+        hipLaunchKernelGGL(k_copy_tiles, dim3(blocks), dim3(256), 0, s, g, g.gv, scratch, dlist, dcount);
+        a = scratch;
+        b = g.gv;
+    }
+    for (uint32_t it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_implicit_sweep, dim3(blocks), dim3(256), 0, s, scenes, g, a, b, dlist, dcount, dt);
+        float4* t = a;
+        a = b;
+        b = t;
+    }
+    return hipGetLastError();
+}
+
 // `also` (may be null) is a second per-cell buffer cleared on the same tiles.
 __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount, float4* also) {
@@ -497,6 +593,11 @@ __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* 
 
 // Resets a counter after every block of the previous launch has read it.
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p, v);
+    return hipGetLastError();
+}
 
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also) {

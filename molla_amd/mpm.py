@@ -99,6 +99,7 @@ _SIGS = {
     "mpm_set_scene_params": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SceneConfig)]),
     "mpm_set_p2g_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_sort_interval": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mpm_set_time_step": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mpm_set_write_vc": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
     "mpm_step_n": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32]),
@@ -186,6 +187,12 @@ class MpmSimulation:
             self._h = None
             raise MpmError(rc, msg)
         self.counts = [0] * len(cfgs)
+        iters = {int(getattr(c.time_step, "jacobi_iters", 0) or 0) for c in cfgs}
+        if len(iters) != 1:
+            self.close()
+            raise MpmError(1, "all scenes of a batch must share one time step")
+        if iters != {0}:
+            self.set_time_step(cfgs[0].time_step)
 
     # -- construction helpers mirroring the reference names --
     @classmethod
@@ -296,6 +303,14 @@ class MpmSimulation:
 
     def set_deterministic_p2g(self, enabled: bool):
         self.set_p2g_mode(P2G_DETERMINISTIC if enabled else P2G_FUSED)
+
+    def set_time_step(self, ts):
+        """``set_time_step`` (mpm.rs:1632): ``Explicit()`` or ``Implicit(jacobi_iters)``."""
+        iters = int(getattr(ts, "jacobi_iters", 0) or 0)
+        self._check(lib().mpm_set_time_step(self._h, iters))
+        for i, c in enumerate(self.cfgs):
+            self.cfgs[i] = c.copy(time_step=ts)
+        self.cfg = self.cfgs[0]
 
     def set_sort_interval(self, every: int):
         self._check(lib().mpm_set_sort_interval(self._h, int(every)))

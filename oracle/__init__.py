@@ -55,6 +55,7 @@ class Scene(C.Structure):
         ("floor_friction", C.c_float),
         ("n_materials", C.c_uint32),
         ("materials", Material * MAX_MATERIALS),
+        ("jacobi_iters", C.c_uint32),
     ]
 
 
@@ -97,6 +98,8 @@ def scene_from(cfg) -> Scene:
     s.model = int(getattr(cfg, "model", MODEL_SHADER))
     s.floor_y = float(getattr(cfg, "floor_y", 0.0))
     s.floor_friction = float(getattr(cfg, "floor_friction", 0.0))
+    ts = getattr(cfg, "time_step", None)
+    s.jacobi_iters = int(getattr(ts, "jacobi_iters", 0) or 0)
     rows = list(cfg.material_rows())
     s.n_materials = len(rows)
     for i, r in enumerate(rows):
@@ -263,6 +266,15 @@ def step(cfg, st: State, dt: float, substeps: int, want_grid=False):
     if rc != 0:
         raise RuntimeError(f"oracle step failed rc={rc}")
     return gm, gp
+
+
+def implicit_sweeps(cfg, dt: float, grid_m, grid_v):
+    """Jacobi sweeps of the semi-implicit step on a dense grid (m: G^3, v: G^3 x 3)."""
+    s = scene_from(cfg)
+    m = np.ascontiguousarray(np.asarray(grid_m, np.float64).reshape(-1))
+    v = np.ascontiguousarray(np.asarray(grid_v, np.float64).reshape(-1, 3).copy())
+    lib().orc_implicit_sweeps(C.byref(s), C.c_double(dt), _p(m, C.c_double), _p(v, C.c_double))
+    return v
 
 
 def p2g(cfg, st: State, dt: float):

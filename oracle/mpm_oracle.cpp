@@ -605,6 +605,55 @@ void grid_update_shader(const orc_scene* s, T dt, Grid<T>& g) {
             }
 }
 
+// mpm_implicit_apply + mpm_implicit_swap: mpm.wgsl:1060-1153, host loop
+// mpm.rs:2515-2530 (no rigid mask).  Each Jacobi sweep blends every cell with
+// m > 1e-9 towards the mean velocity of its face neighbours with m > 1e-9:
+// v' = (m v + a v_avg) / (m + a), a = dt^2 m (E / 1500) / dx^2; cells without
+// such neighbours keep v; empty cells are copied.  Sweeps read the previous
+// sweep's velocities (ping-pong).
+template <typename T>
+void implicit_sweeps(const orc_scene* s, T dt, Grid<T>& g) {
+    const int n = int(g.G);
+    const T inv_dx = T(1) / T(s->cell_size);
+    const T c2 = T(s->youngs) / T(1500);
+    std::vector<T> nv(g.v.size());
+    for (uint32_t it = 0; it < s->jacobi_iters; ++it) {
+        for (int iz = 0; iz < n; ++iz)
+            for (int iy = 0; iy < n; ++iy)
+                for (int ix = 0; ix < n; ++ix) {
+                    const size_t c = g.idx(ix, iy, iz);
+                    const T m = g.m[c];
+                    if (m <= T(1.0e-9) || !(m == m)) {
+                        for (int a = 0; a < 3; ++a) nv[3 * c + a] = g.v[3 * c + a];
+                        continue;
+                    }
+                    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+                    T avg[3] = {T(0), T(0), T(0)};
+                    T cnt = T(0);
+                    for (int k = 0; k < 6; ++k) {
+                        const int jx = ix + off[k][0], jy = iy + off[k][1], jz = iz + off[k][2];
+                        if (!g.in(jx, jy, jz)) continue;
+                        const size_t d = g.idx(jx, jy, jz);
+                        const T nm = g.m[d];
+                        if (nm <= T(1.0e-9) || !(nm == nm)) continue;
+                        for (int a = 0; a < 3; ++a) avg[a] += g.v[3 * d + a];
+                        cnt += T(1);
+                    }
+                    for (int a = 0; a < 3; ++a) avg[a] = cnt > T(0) ? avg[a] / cnt : g.v[3 * c + a];
+                    const T alpha = dt * dt * m * c2 * inv_dx * inv_dx;
+                    const T den = m + alpha;
+                    T out[3];
+                    bool fin = true;
+                    for (int a = 0; a < 3; ++a) {
+                        out[a] = (m * g.v[3 * c + a] + alpha * avg[a]) / den;
+                        fin = fin && out[a] == out[a];
+                    }
+                    for (int a = 0; a < 3; ++a) nv[3 * c + a] = fin ? out[a] : g.v[3 * c + a];
+                }
+        g.v.swap(nv);
+    }
+}
+
 // g2p: mpm.wgsl:1241-1353.  Returns false (and leaves the particle
 // untouched) on a numerical fault, mirroring the shader's early return.
 template <typename T>
@@ -721,6 +770,7 @@ int step_shader(const orc_scene* s, Particles<T>& P, T dt, uint32_t substeps, ui
         if (out_m && st + 1 == substeps) std::copy(g.m.begin(), g.m.end(), out_m);
         if (out_p && st + 1 == substeps) std::copy(g.p.begin(), g.p.end(), out_p);
         grid_update_shader(s, dt, g);
+        implicit_sweeps(s, dt, g);
         if (fault == 0) {
             if (P.n > 0) gathers++;
             uint32_t f = 0;
@@ -1070,6 +1120,17 @@ int step_dispatch(const orc_scene* s, uint32_t n, T* x, T* v, T* C, T* F, T* jp,
 }  // namespace
 
 extern "C" {
+// implicit_sweeps on a caller-supplied dense grid (m[G^3], v[3 G^3], x
+// fastest), for unit tests of the sweep itself.
+void orc_implicit_sweeps(const orc_scene* s, double dt, const double* m, double* v) {
+    Grid<double> g;
+    g.init(s->grid_size);
+    std::copy(m, m + g.m.size(), g.m.begin());
+    std::copy(v, v + g.v.size(), g.v.begin());
+    implicit_sweeps(s, dt, g);
+    std::copy(g.v.begin(), g.v.end(), v);
+}
+
 
 // tile_keys: shaders/mpm_particle_order.wgsl:13-25 and the closed-form key()
 // of src/mpm_particle_order_tests.rs:34-43: cell = floor((x - o) * inv_dx)

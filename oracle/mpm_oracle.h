@@ -49,6 +49,7 @@ typedef struct {
     float floor_y, floor_friction; /* CPUDIFF model only */
     uint32_t n_materials;
     orc_material materials[ORC_MAX_MATERIALS];
+    uint32_t jacobi_iters; /* MpmTimeStep::Implicit sweeps (0 = explicit), shader model */
 } orc_scene;
 
 /* Fault bits (reference g_kdiag tail metadata, mpm.wgsl:175-231). */
@@ -75,6 +76,9 @@ void orc_quad_weights_dfx(double fx, double* dw);
 /* Plasticity return maps on singular values (mpm.wgsl:1355-1479). */
 void orc_plastic_project(const orc_scene* s, const orc_material* m, const double* sigma,
                          double jp_in, double* sigma_out, double* jp_out);
+
+/* Semi-implicit Jacobi sweeps (scene jacobi_iters) on a dense grid. */
+void orc_implicit_sweeps(const orc_scene* s, double dt, const double* grid_m, double* grid_v);
 
 /* ---- full substeps ---- */
 /* Runs `substeps` substeps in place.  E may be NULL (scene youngs used).

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from molla_amd import Fluid, MpmConfig, MpmSimulation, Sand, Snow, Viscous, workloads
+from molla_amd import Fluid, Implicit, MpmConfig, MpmSimulation, Sand, Snow, Viscous, workloads
 from tests.fixtures import fill_box, order_config, points
 from tests.gpu_util import log, oracle_state, rel_err, run_gpu
 
@@ -311,3 +311,58 @@ def test_autotune_restores_state_and_keeps_results():
     log(f"  tuned vs default x: {e:.2e}")
     assert e < 1e-6
     assert sim.read_status()["completed_gathers"] == 30
+
+
+# ------------------------------------------------------------ semi-implicit
+@pytest.mark.parametrize("iters", [1, 2, 3])
+def test_implicit_step_matches_fp64_oracle(iters):
+    # MpmTimeStep::Implicit (mpm.wgsl:1060-1153): the Jacobi sweeps on the
+    # active tiles against the oracle's dense-grid sweeps.
+    cfg = MpmConfig.sand_calibrated().copy(time_step=Implicit(iters), rheology=Viscous(0, 0))
+    x = fill_box([-0.2, 0.3, -0.2], [0.2, 0.6, 0.2], 10, 41)
+    errs = _compare_with_oracle(cfg, x, 1e-3, 40, tag=f"implicit{iters}")
+    _assert_errs(errs, {"x": 1e-6, "v": 1e-5, "F": 1e-5, "C": 1e-4})
+
+
+def test_implicit_vs_explicit_settle_like_the_reference():
+    # tests/mpm_v3_gpu.rs:160-230: a sand pile settles under both time steps;
+    # per-particle paths stay within 0.20 m and the pile has dropped.
+    x = fill_box([-0.10, 0.30, -0.10], [0.10, 0.50, 0.10], 6, 1234)
+    dt = 1.0 / 480.0
+    out = {}
+    for name, ts in (("explicit", None), ("implicit", Implicit(2))):
+        cfg = MpmConfig.sand().copy(youngs=2.0e4)
+        if ts is not None:
+            cfg = cfg.copy(time_step=ts)
+        sim = MpmSimulation(cfg, max_particles=len(x))
+        sim.set_particles(x, None)
+        for _ in range(480):
+            sim.step_n(dt, 1)
+        out[name] = sim.read_positions()
+        assert sim.read_status()["fault_bits"] == 0
+        sim.close()
+    diff = np.linalg.norm(out["explicit"] - out["implicit"], axis=1)
+    y0 = x[:, 1].mean()
+    log(f"implicit vs explicit: max diff {diff.max():.4f} m, mean y {y0:.3f} -> "
+        f"{out['explicit'][:, 1].mean():.3f} / {out['implicit'][:, 1].mean():.3f}")
+    assert out["explicit"][:, 1].mean() < y0 - 0.05
+    assert diff.max() < 0.20
+
+
+def test_implicit_survives_larger_dt_like_the_reference():
+    # tests/mpm_v3_gpu.rs:234-312: at 3x the explicit CFL limit the implicit
+    # step stays finite and bounded (< 100 m/s).
+    x = fill_box([-0.10, 0.60, -0.10], [0.10, 0.80, 0.10], 6, 555)
+    v = np.tile(np.array([0.0, -3.0, 0.0], np.float32), (len(x), 1))
+    cfg = MpmConfig.sand().copy(youngs=5.0e5, time_step=Implicit(5))
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x, v)
+    for _ in range(40):
+        sim.step_n(1.0 / 133.0, 1)
+    pos, vel = sim.read_positions(), sim.read_velocities()
+    st = sim.read_status()
+    sim.close()
+    speed = float(np.linalg.norm(vel, axis=1).max())
+    log(f"implicit at dt=1/133: max |v| {speed:.2f} m/s, fault bits {st['fault_bits']}")
+    assert st["fault_bits"] == 0 and np.all(np.isfinite(pos))
+    assert speed < 100.0

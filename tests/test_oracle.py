@@ -42,7 +42,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from molla_amd.config import (MODEL_CPUDIFF, Fluid, MpmConfig, Sand, Snow, Viscous,
+from molla_amd.config import (MODEL_CPUDIFF, Fluid, Implicit, MpmConfig, Sand, Snow, Viscous,
                               drucker_prager_alpha, material_row)
 from tests.fixtures import fill_box, order_config, points, reference_key
 
@@ -562,3 +562,60 @@ def test_adjoint_full_central_fd_x0_v0_and_per_particle_E():
         checks.append((out["E"][p], fde))
     for an, fd in checks:
         assert abs(an - fd) <= 1e-5 * max(abs(an), abs(fd)) + 1e-10, (an, fd)
+
+
+def test_implicit_sweep_closed_forms():
+    # mpm_implicit_apply / mpm_implicit_swap (mpm.wgsl:1060-1153):
+    # v' = (m v + a v_avg) / (m + a), a = dt^2 m (E / 1500) / dx^2, v_avg over
+    # face neighbours with m > 1e-9; sweeps ping-pong.
+    G, dt, E, dx = 8, 1e-2, 2.0e4, 0.1
+    cfg1 = MpmConfig.sand().copy(grid_size=G, cell_size=dx, youngs=E, time_step=Implicit(1))
+    cfg2 = cfg1.copy(time_step=Implicit(2))
+
+    def idx(x, y, z):
+        return x + G * (y + G * z)
+
+    m = np.zeros(G ** 3)
+    v = np.zeros((G ** 3, 3))
+    iso, a_, b_, tiny = idx(1, 1, 1), idx(4, 4, 4), idx(5, 4, 4), idx(4, 5, 4)
+    m[iso], v[iso] = 0.5, (1.0, 2.0, 3.0)
+    m[a_], v[a_] = 0.2, (1.0, 0.0, 0.0)
+    m[b_], v[b_] = 0.8, (-1.0, 0.5, 0.0)
+    m[tiny], v[tiny] = 1e-10, (100.0, 100.0, 100.0)  # below the mass threshold: ignored, copied
+
+    dx32 = float(np.float32(dx))  # the scene stores cell_size as f32
+
+    def alpha(mm):
+        return dt * dt * mm * (E / 1500.0) / (dx32 * dx32)
+
+    def one(va, vb):
+        return ((m[a_] * va + alpha(m[a_]) * vb) / (m[a_] + alpha(m[a_])),
+                (m[b_] * vb + alpha(m[b_]) * va) / (m[b_] + alpha(m[b_])))
+
+    out1 = orc.implicit_sweeps(cfg1, dt, m, v)
+    ea, eb = one(v[a_], v[b_])
+    np.testing.assert_allclose(out1[a_], ea, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(out1[b_], eb, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(out1[iso], v[iso], rtol=1e-14)  # v_avg = v: unchanged up to rounding
+    np.testing.assert_array_equal(out1[tiny], v[tiny])
+    out2 = orc.implicit_sweeps(cfg2, dt, m, v)
+    ea2, eb2 = one(ea, eb)
+    np.testing.assert_allclose(out2[a_], ea2, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(out2[b_], eb2, rtol=1e-13, atol=1e-15)
+    # momentum of the pair moves towards the common velocity, never past it
+    assert np.all(np.abs(out1[a_] - out1[b_]) <= np.abs(v[a_] - v[b_]) + 1e-15)
+
+
+def test_implicit_step_keeps_uniform_translation():
+    # A uniform grid velocity is a fixed point of the sweep (v_avg = v), so the
+    # semi-implicit step moves a freely translating block like the explicit one.
+    cfg = MpmConfig.sand().copy(gravity_y=0.0, youngs=2.0e4, boundary_friction=1.0,
+                                rheology=Viscous(0, 0))
+    x = fill_box([-0.15, 0.3, -0.15], [0.15, 0.6, 0.15], 8, 3)
+    v = np.tile(np.array([0.2, -0.1, 0.05]), (len(x), 1))
+    se = orc.State(x, v, dtype=np.float64)
+    si = orc.State(x, v, dtype=np.float64)
+    orc.step(cfg, se, 1e-3, 20)
+    orc.step(cfg.copy(time_step=Implicit(3)), si, 1e-3, 20)
+    assert np.abs(se.x - si.x).max() < 1e-12
+    assert np.abs(se.v - si.v).max() < 1e-10
