@@ -218,6 +218,91 @@ int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad
  * VELOCITY, C, F (row-major 3x3) or YOUNGS. */
 int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n);
 
+/* ---- soil particle checkpoint, "MOLLSOIL" v1/v2 (mpm_checkpoint.rs) ----
+ * Byte layout (little-endian, MpmSoilCheckpoint::to_bytes, :185-214):
+ *   [0, 8)   "MOLLSOIL"                [8, 12)  version: 1, or 2 when the
+ *   [12, 44) BLAKE3 of the payload              grid window is non-zero
+ *            (v2: of version bytes || payload, checkpoint_digest :87-94)
+ *   payload  u32 words: [window x3 (v2 only)] config (21) n_materials
+ *            materials (4 each) count, then 40 words per particle in nine
+ *            streams of STRIDES = 4,4,12,12,1,1,1,1,4 words (:78-83):
+ *            position+w, velocity+w, C and F as three padded columns
+ *            (column-major, word 4j+i = M_ij), volume, mass, log-Jp,
+ *            material id (u32), tire compaction.
+ * The codec below is host-only and needs no handle; its messages are read
+ * with mpm_soil_last_error(). */
+#define MPM_SOIL_HEADER_BYTES 44        /* HEADER_BYTES, :81       */
+#define MPM_SOIL_WORDS_PER_PARTICLE 40  /* WORDS_PER_PARTICLE, :83 */
+#define MPM_SOIL_CONFIG_WORDS 21        /* config_words, :594-619  */
+
+/* MpmSoilCheckpoint (:137-147) minus its particle words.  `config` is
+ * config_words(MpmConfig): grid_size, max_particles, max_rigid, then the
+ * f32 bits of cell_size, grid_origin[3], gravity_y, youngs, poisson,
+ * friction_angle, cohesion, boundary_friction, particle_mass,
+ * particle_volume, then the rheology (4 words, rheology_words :544-557:
+ * kind 0 sand, 1 snow, 2 viscous, then its f32 parameters) and the time
+ * step (0,0 explicit; 1,iters implicit). */
+typedef struct {
+    int32_t grid_window_cells[3];
+    uint32_t config[MPM_SOIL_CONFIG_WORDS];
+    uint32_t n_materials; /* 0: the config's rheology is authoritative */
+    uint32_t materials[MPM_MAX_MATERIALS][4];
+    uint32_t count;       /* particles; words hold 40 * count u32 */
+} mpm_soil_meta;
+
+/* Encoded size of a checkpoint with this metadata. */
+uint64_t mpm_soil_encoded_bytes(const mpm_soil_meta* meta);
+/* MpmSoilCheckpoint::to_bytes (:185-214): no semantic validation, like the
+ * reference; `cap` must hold mpm_soil_encoded_bytes(meta). */
+int mpm_soil_encode(const mpm_soil_meta* meta, const uint32_t* words, uint8_t* out, uint64_t cap);
+/* MpmSoilCheckpoint::from_bytes (:216-267) up to, not including, the copy
+ * of the particle words and validate(): format, version, allocation limit
+ * (`max_particles`), magic, digest, canonical metadata, count and payload
+ * length.  Fills `meta` so the caller can size the word buffer. */
+int mpm_soil_decode_meta(const uint8_t* bytes, uint64_t len, uint32_t max_particles, mpm_soil_meta* meta);
+/* The whole of from_bytes: as above, then validate() and copy the particle
+ * words (words_cap >= 40 * count). */
+int mpm_soil_decode(const uint8_t* bytes, uint64_t len, uint32_t max_particles, mpm_soil_meta* meta,
+                    uint32_t* words, uint64_t words_cap);
+/* MpmSoilCheckpoint::validate (:274-330): configuration ranges, rheology
+ * parameters, finite particle history, positive mass and volume, material
+ * ids inside the table. */
+int mpm_soil_validate(const mpm_soil_meta* meta, const uint32_t* words);
+/* validate_lattice_window (mpm_soil_lattice_window.rs:6-39, with
+ * validate_node_window, soil_candidates_gpu.rs:68-94): every particle's
+ * stencil lies inside the integer grid window; checked on restore of a
+ * non-zero window. */
+int mpm_soil_validate_window(const mpm_soil_meta* meta, const uint32_t* words);
+/* The digest primitive (blake3::hash, 32 bytes). */
+int mpm_blake3(const void* data, uint64_t len, uint8_t out[32]);
+/* Message of the calling thread's last codec failure. */
+const char* mpm_soil_last_error(void);
+
+/* Device side of checkpoint_soil_with (:388-431, with encode_checkpoint_copy
+ * :356-372 and CheckpointMetadata::finish :103-122): the nine streams of all
+ * `n` particles (n = the particle count) in slot order, one gather kernel and
+ * one device-to-host copy.  Single-scene MPM_MODEL_SHADER handles only.
+ * Fails with MPM_ERR_FAULT while a grid fault is latched, and after
+ * mpm_reset_status() of a latched fault until a step or restore has run
+ * (an incomplete state cannot be archived). */
+int mpm_checkpoint_capture(mpm_sim* sim, uint32_t* words, uint32_t n);
+/* Device side of restore_soil_with (:439-541: validate_soil_restore_origin,
+ * encode_soil_restore and finish_soil_restore).  `meta` and `words` are a
+ * decoded checkpoint; `rows` is its material table packed as material_row
+ * (mpm.rs:666-686) does, n_rows = max(meta->n_materials, 1).  The call
+ * validates the checkpoint (and its window when non-zero), checks that it
+ * fits the handle's capacity and that every configuration field the handle
+ * holds matches (grid size, cell size, origin, gravity, E, nu, wall
+ * friction, particle mass and volume, time step), then replaces the particle
+ * set, the material table and the grid window in one upload, clears the grid
+ * and the fault latch and drops captured step graphs.  Nothing changes when
+ * it fails.  Fields the handle does not hold (max_rigid, friction_angle,
+ * cohesion, the config rheology) are compared by the caller, which owns the
+ * MpmConfig.  Per-particle E, which the format does not carry, is reset to
+ * the scene's E. */
+int mpm_checkpoint_restore(mpm_sim* sim, const mpm_soil_meta* meta, const uint32_t* words,
+                           const mpm_material* rows, uint32_t n_rows);
+
 #ifdef __cplusplus
 }
 #endif

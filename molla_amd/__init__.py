@@ -6,7 +6,8 @@
 from .config import (MODEL_CPUDIFF, MODEL_SHADER, Explicit, Fluid, Implicit, MpmConfig,
                      Sand, Snow, Viscous, material_row)
 from .mpm import (P2G_DETERMINISTIC, P2G_FUSED, P2G_GLOBAL_ATOMIC, MpmError, MpmSimulation)
+from .checkpoint import CheckpointError, MpmSoilCheckpoint
 
 __all__ = ["MpmConfig", "Sand", "Snow", "Viscous", "Fluid", "Explicit", "Implicit",
            "MODEL_SHADER", "MODEL_CPUDIFF", "material_row", "MpmSimulation", "MpmError",
-           "P2G_FUSED", "P2G_GLOBAL_ATOMIC", "P2G_DETERMINISTIC"]
+           "P2G_FUSED", "P2G_GLOBAL_ATOMIC", "P2G_DETERMINISTIC", "MpmSoilCheckpoint", "CheckpointError"]

@@ -19,7 +19,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmolla_mpm.so")
-SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_scan.hip", "mpm_capi.hip", "mpm_diff.hip"]
+SOURCES = ["mpm_kernels.hip", "mpm_sort.hip", "mpm_scan.hip", "mpm_capi.hip", "mpm_diff.hip", "mpm_soil_codec.cpp",
+           "blake3.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

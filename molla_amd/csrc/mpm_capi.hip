@@ -124,6 +124,13 @@ struct mpm_sim {
     float4* gadj = nullptr;    // per-cell grid adjoint
     double* gml = nullptr;     // per-chunk (dJ/dmu, dJ/dlambda)
     bool adj_valid = false;
+    // soil checkpoint: the archived words the planes do not hold (position and
+    // velocity w, C/F column pads, tire compaction; 12 per particle, by id)
+    uint32_t* ck_aux = nullptr;
+    bool aux_live = false;     // ck_aux holds a restored set (else those words are zero)
+    bool w_stepped = false;    // a step ran since: position/velocity w read 0 (mpm.wgsl:1349-1350)
+    bool fault_reset = false;  // a latched fault was cleared by mpm_reset_status: no archive
+                               // until a step or a restore (a reset does not hide the fault)
     std::string err;
 };
 
@@ -185,6 +192,11 @@ std::string validate(const mpm_scene_config& c) {
     return "";
 }
 
+void drop_graphs(mpm_sim* s) {
+    for (auto& kv : s->graphs) hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+}
+
 template <typename T>
 hipError_t dalloc(T** p, size_t count) {
     *p = nullptr;
@@ -210,7 +222,7 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2};
+    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux};
     for (void* b : dbufs)
         if (b) hipFree(b);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -468,6 +480,8 @@ int deterministic_p2g(mpm_sim* s, float dt) {
 // The substep sequence of one step_n call.
 int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
     int rc;
+    s->w_stepped = true;
+    s->fault_reset = false;
     if (!s->dry && (rc = ensure_grid(s)) != MPM_OK) return rc;
     if ((rc = clear_grid(s)) != MPM_OK) return rc;
     const bool det = s->p2g_mode == MPM_P2G_DETERMINISTIC;
@@ -676,6 +690,8 @@ int set_particles(mpm_sim* s, const uint32_t* counts, const float* pos, const fl
     s->has_E = 0;
     s->sorted = false;
     s->diff_steps = 0;
+    s->aux_live = false;
+    drop_graphs(s);  // captured steps bake in the particle count and chunk table
     if (s->gmp) {
         rc = clear_grid(s);
         if (rc != MPM_OK) return rc;
@@ -1113,7 +1129,7 @@ int mpm_set_materials(mpm_sim* sim, uint32_t scene, const mpm_material* rows, ui
     sim->cfg[scene].n_materials = n_rows;
     sim->sh[scene].n_mat = n_rows;
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
-    sim->graphs.clear();
+    drop_graphs(sim);
     return upload_scenes(sim);
 }
 
@@ -1276,6 +1292,9 @@ int mpm_read_status(mpm_sim* sim, mpm_status_info* out) {
 int mpm_reset_status(mpm_sim* sim) {
     if (!sim) return MPM_ERR_INVALID;
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    uint32_t fault = 0;
+    HIP_TRY(sim, hipMemcpy(&fault, sim->status + mpm::ST_FAULT, 4, hipMemcpyDeviceToHost));
+    if (fault) sim->fault_reset = true;
     uint32_t z[2] = {0u, 0u};
     HIP_TRY(sim, hipMemcpy(sim->status, z, sizeof(z), hipMemcpyHostToDevice));
     HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_ESCAPES, z, 4, hipMemcpyHostToDevice));
@@ -1365,6 +1384,122 @@ int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n) {
     if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "output length must equal the particle count");
     if (!out) return fail(sim, MPM_ERR_INVALID, "out must not be NULL");
     return read_adjoint(sim, field, out);
+}
+
+// ---- soil checkpoint (mpm_checkpoint.rs) ----------------------------------
+
+namespace {
+
+int soil_handle(mpm_sim* s) {
+    if (s->n_scenes != 1 || s->cfg[0].model != MPM_MODEL_SHADER)
+        return fail(s, MPM_ERR_INVALID, "the soil checkpoint is defined for single-scene MPM_MODEL_SHADER handles");
+    return MPM_OK;
+}
+
+int soil_fail(mpm_sim* s) { return fail(s, MPM_ERR_INVALID, mpm_soil_last_error()); }
+
+// A device buffer that lives for one archive operation.
+struct TempWords {
+    uint32_t* p = nullptr;
+    ~TempWords() {
+        if (p) hipFree(p);
+    }
+};
+
+}  // namespace
+
+int mpm_checkpoint_capture(mpm_sim* sim, uint32_t* words, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int rc = soil_handle(sim);
+    if (rc != MPM_OK) return rc;
+    if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "n must equal the particle count");
+    if (n && !words) return fail(sim, MPM_ERR_INVALID, "words must not be NULL");
+    HIP_TRY(sim, hipSetDevice(sim->device));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    uint32_t fault = 0;
+    HIP_TRY(sim, hipMemcpy(&fault, sim->status + mpm::ST_FAULT, 4, hipMemcpyDeviceToHost));
+    if (fault || sim->fault_reset)
+        return fail(sim, MPM_ERR_FAULT,
+                    "MPM soil checkpoint: latched grid fault; incomplete state cannot be archived");
+    if (n == 0) return MPM_OK;
+    TempWords d;
+    const size_t nw = (size_t)MPM_SOIL_WORDS_PER_PARTICLE * n;
+    HIP_TRY(sim, dalloc(&d.p, nw));
+    HIP_TRY(sim, mpm::launch_ckpt_pack(sim->ps[sim->ps_cur], sim->cap, sim->slot_of_id,
+                                       sim->aux_live ? sim->ck_aux : nullptr, sim->w_stepped ? 1 : 0, n, d.p,
+                                       sim->stream));
+    HIP_TRY(sim, hipMemcpyAsync(words, d.p, nw * 4, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    return MPM_OK;
+}
+
+int mpm_checkpoint_restore(mpm_sim* sim, const mpm_soil_meta* meta, const uint32_t* words,
+                           const mpm_material* rows, uint32_t n_rows) {
+    if (!sim) return MPM_ERR_INVALID;
+    int rc = soil_handle(sim);
+    if (rc != MPM_OK) return rc;
+    if (!meta || !rows) return fail(sim, MPM_ERR_INVALID, "meta and rows must not be NULL");
+    if (n_rows != std::max<uint32_t>(meta->n_materials, 1) || n_rows > MPM_MAX_MATERIALS)
+        return fail(sim, MPM_ERR_INVALID, "rows must pack the checkpoint's material table (max(n_materials, 1) rows)");
+    if (mpm_soil_validate(meta, words) != MPM_OK) return soil_fail(sim);
+    const int32_t* win = meta->grid_window_cells;
+    if ((win[0] || win[1] || win[2]) && mpm_soil_validate_window(meta, words) != MPM_OK) return soil_fail(sim);
+    const mpm_scene_config& c = sim->cfg[0];
+    auto bits = [](float f) {
+        uint32_t w;
+        std::memcpy(&w, &f, 4);
+        return w;
+    };
+    const uint32_t* k = meta->config;
+    const uint32_t iters = k[19] == 1 ? k[20] : 0;
+    const bool same = k[0] == c.grid_size && k[3] == bits(c.cell_size) && k[4] == bits(c.grid_origin[0]) &&
+                      k[5] == bits(c.grid_origin[1]) && k[6] == bits(c.grid_origin[2]) &&
+                      k[7] == bits(c.gravity_y) && k[8] == bits(c.youngs) && k[9] == bits(c.poisson) &&
+                      k[12] == bits(c.boundary_friction) && k[13] == bits(c.particle_mass) &&
+                      k[14] == bits(c.particle_volume) && iters == sim->jacobi_iters;
+    if (!same || meta->count > c.max_particles || meta->count > sim->cap)
+        return fail(sim, MPM_ERR_INVALID,
+                    "MPM soil checkpoint: incompatible configuration, capacity or attached rigid bodies");
+    // validated: from here on the handle is replaced
+    HIP_TRY(sim, hipSetDevice(sim->device));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    const uint32_t n = meta->count;
+    if (!sim->ck_aux) HIP_TRY(sim, dalloc(&sim->ck_aux, (size_t)12 * sim->cap));
+    TempWords d;
+    if (n) {
+        const size_t nw = (size_t)MPM_SOIL_WORDS_PER_PARTICLE * n;
+        HIP_TRY(sim, dalloc(&d.p, nw));
+        HIP_TRY(sim, hipMemcpyAsync(d.p, words, nw * 4, hipMemcpyHostToDevice, sim->stream));
+        HIP_TRY(sim, mpm::launch_ckpt_unpack(d.p, n, sim->ps[0], sim->cap, c.youngs, sim->ck_aux, sim->stream));
+    }
+    mpm_scene_config nc = c;
+    for (int a = 0; a < 3; ++a) nc.grid_window_cells[a] = win[a];
+    for (uint32_t r = 0; r < n_rows; ++r) nc.materials[r] = rows[r];
+    nc.n_materials = n_rows;
+    sim->cfg[0] = nc;
+    sim->sh[0] = make_scene(nc, sim->sh[0].tile_base, sim->sh[0].mat_base);
+    sim->n = n;
+    sim->counts.assign(1, n);
+    sim->pbase.assign(1, 0);
+    sim->ps_cur = 0;
+    HIP_TRY(sim, mpm::launch_index_slots(sim->ps[0], sim->cap, sim->slot_of_id, n, sim->stream));
+    if ((rc = build_chunks(sim)) != MPM_OK) return rc;
+    if ((rc = upload_scenes(sim)) != MPM_OK) return rc;
+    if (sim->gmp && (rc = clear_grid(sim)) != MPM_OK) return rc;
+    const uint32_t z[2] = {0u, 0u};
+    HIP_TRY(sim, hipMemcpyAsync(sim->status + mpm::ST_FAULT, z, sizeof(z), hipMemcpyHostToDevice, sim->stream));
+    HIP_TRY(sim, hipMemcpyAsync(sim->status + mpm::ST_ESCAPES, z, 4, hipMemcpyHostToDevice, sim->stream));
+    HIP_TRY(sim, hipMemcpyAsync(sim->status + mpm::ST_FALLBACK, z, 4, hipMemcpyHostToDevice, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    sim->has_E = 0;
+    sim->sorted = false;
+    sim->diff_steps = 0;
+    sim->adj_valid = false;
+    sim->aux_live = true;
+    sim->w_stepped = false;
+    sim->fault_reset = false;
+    drop_graphs(sim);
+    return MPM_OK;
 }
 
 }  // extern "C"

@@ -171,6 +171,12 @@ hipError_t launch_unpermute(const float* ps, uint32_t cap, int comp0, int ncomp,
 hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
                              const uint32_t* slot_of_id, const float* in, uint32_t n,
                              hipStream_t s);
+// Soil checkpoint: planes -> 40-word streams (ids in original order), and
+// streams -> planes in identity slot order; aux = 12 words per particle.
+hipError_t launch_ckpt_pack(const float* ps, uint32_t cap, const uint32_t* slot_of_id, const uint32_t* aux,
+                            int zero_w, uint32_t n, uint32_t* out, hipStream_t s);
+hipError_t launch_ckpt_unpack(const uint32_t* in, uint32_t n, float* ps, uint32_t cap, float youngs,
+                              uint32_t* aux, hipStream_t s);
 hipError_t launch_grid_to_dense(const float4* gmp, const SceneDev* scenes, uint32_t scene,
                                 uint32_t G, float* mass, float* mom, hipStream_t s);
 // Deterministic gather-order P2G.

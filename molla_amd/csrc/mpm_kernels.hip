@@ -798,6 +798,89 @@ hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
     return hipGetLastError();
 }
 
+// ---- soil checkpoint streams (mpm_checkpoint.rs:78-83, :356-372) ----------
+// The archive holds n particles as nine streams, stream-major, 40 words per
+// particle: position+w (4), velocity+w (4), C (12), F (12), volume, mass,
+// log-Jp, material id, tire compaction (4).  C and F are three 16-byte
+// columns, word 4j+i = M_ij, word 4j+3 a pad.  The planes hold M row-major
+// (plane 3i+j).  `aux` keeps, by particle id, the words the planes do not:
+// position w [n], velocity w [n], C pads [3n], F pads [3n], tire [4n]; a null
+// aux reads as zeros.  Every stream offset is a multiple of 4 words from a
+// 256-byte-aligned base, so the 4-word groups move as 16-byte accesses.
+__global__ void k_ckpt_pack(const float* __restrict__ ps, uint32_t cap, const uint32_t* __restrict__ slot_of_id,
+                            const uint32_t* __restrict__ aux, int zero_w, uint32_t n, uint32_t* __restrict__ out) {
+    const size_t N = n;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const size_t s = slot_of_id[p];
+        auto P = [&](int c) { return __float_as_uint(ps[(size_t)c * cap + s]); };
+        auto A = [&](size_t i) { return aux ? aux[i] : 0u; };
+        // a step stores w = 0 (mpm.wgsl:1349-1350)
+        reinterpret_cast<uint4*>(out)[p] = make_uint4(P(CX), P(CX + 1), P(CX + 2), zero_w ? 0u : A(p));
+        reinterpret_cast<uint4*>(out + 4 * N)[p] = make_uint4(P(CV), P(CV + 1), P(CV + 2), zero_w ? 0u : A(N + p));
+        for (int j = 0; j < 3; ++j) {
+            reinterpret_cast<uint4*>(out + 8 * N)[3 * p + j] =
+                make_uint4(P(CC + j), P(CC + 3 + j), P(CC + 6 + j), A(2 * N + 3 * p + j));
+            reinterpret_cast<uint4*>(out + 20 * N)[3 * p + j] =
+                make_uint4(P(CF + j), P(CF + 3 + j), P(CF + 6 + j), A(5 * N + 3 * p + j));
+        }
+        out[32 * N + p] = P(CVOL);
+        out[33 * N + p] = P(CMASS);
+        out[34 * N + p] = P(CJP);
+        out[35 * N + p] = P(CMAT);
+        const size_t t = 8 * N + 4 * (size_t)p;
+        reinterpret_cast<uint4*>(out + 36 * N)[p] = make_uint4(A(t), A(t + 1), A(t + 2), A(t + 3));
+    }
+}
+
+// Inverse of k_ckpt_pack into slot = id order; E is the scene's, the id plane
+// the identity.  `aux` (12n words) receives the words the planes do not hold.
+__global__ void k_ckpt_unpack(const uint32_t* __restrict__ in, uint32_t n, float* __restrict__ ps, uint32_t cap,
+                              float youngs, uint32_t* __restrict__ aux) {
+    const size_t N = n;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        auto put = [&](int c, uint32_t w) { ps[(size_t)c * cap + p] = __uint_as_float(w); };
+        const uint4 x = reinterpret_cast<const uint4*>(in)[p];
+        const uint4 v = reinterpret_cast<const uint4*>(in + 4 * N)[p];
+        put(CX, x.x), put(CX + 1, x.y), put(CX + 2, x.z);
+        put(CV, v.x), put(CV + 1, v.y), put(CV + 2, v.z);
+        aux[p] = x.w;
+        aux[N + p] = v.w;
+        for (int j = 0; j < 3; ++j) {
+            const uint4 c = reinterpret_cast<const uint4*>(in + 8 * N)[3 * p + j];
+            const uint4 f = reinterpret_cast<const uint4*>(in + 20 * N)[3 * p + j];
+            put(CC + j, c.x), put(CC + 3 + j, c.y), put(CC + 6 + j, c.z);
+            put(CF + j, f.x), put(CF + 3 + j, f.y), put(CF + 6 + j, f.z);
+            aux[2 * N + 3 * p + j] = c.w;
+            aux[5 * N + 3 * p + j] = f.w;
+        }
+        put(CVOL, in[32 * N + p]);
+        put(CMASS, in[33 * N + p]);
+        put(CJP, in[34 * N + p]);
+        put(CMAT, in[35 * N + p]);
+        const uint4 t = reinterpret_cast<const uint4*>(in + 36 * N)[p];
+        const size_t a = 8 * N + 4 * (size_t)p;
+        aux[a] = t.x, aux[a + 1] = t.y, aux[a + 2] = t.z, aux[a + 3] = t.w;
+        ps[(size_t)CE * cap + p] = youngs;
+        put(CID, p);
+    }
+}
+
+hipError_t launch_ckpt_pack(const float* ps, uint32_t cap, const uint32_t* slot_of_id, const uint32_t* aux,
+                            int zero_w, uint32_t n, uint32_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = std::min<unsigned>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_ckpt_pack, dim3(blocks), dim3(256), 0, s, ps, cap, slot_of_id, aux, zero_w, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ckpt_unpack(const uint32_t* in, uint32_t n, float* ps, uint32_t cap, float youngs,
+                              uint32_t* aux, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned blocks = std::min<unsigned>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_ckpt_unpack, dim3(blocks), dim3(256), 0, s, in, n, ps, cap, youngs, aux);
+    return hipGetLastError();
+}
+
 __global__ void k_grid_to_dense(const float4* __restrict__ gmp, const SceneDev* __restrict__ scenes,
                                 uint32_t scene, uint32_t G, float* __restrict__ mass, float* __restrict__ mom) {
     const SceneDev sc = scenes[scene];

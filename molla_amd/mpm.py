@@ -127,7 +127,9 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise MpmError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build()")
         L = C.CDLL(LIB_PATH)
-        for name, (res, args) in _SIGS.items():
+        from .checkpoint import SIGS as _CKPT_SIGS
+
+        for name, (res, args) in {**_SIGS, **_CKPT_SIGS}.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -136,7 +138,9 @@ def lib():
 
 
 def exported_symbols() -> List[str]:
-    return list(_SIGS)
+    from .checkpoint import SIGS as _CKPT_SIGS
+
+    return list(_SIGS) + list(_CKPT_SIGS)
 
 
 def scene_config(cfg: MpmConfig, max_particles: Optional[int] = None) -> SceneConfig:
@@ -173,6 +177,8 @@ class MpmSimulation:
 
     def __init__(self, cfg, max_particles: Optional[int] = None):
         cfgs = list(cfg) if isinstance(cfg, (list, tuple)) else [cfg]
+        if max_particles is not None:  # the capacity is cfg.max_particles, as in the reference
+            cfgs = [c.copy(max_particles=int(max_particles)) for c in cfgs]
         self.cfgs: List[MpmConfig] = cfgs
         self.cfg = cfgs[0]
         arr = (SceneConfig * len(cfgs))(*[scene_config(c, max_particles) for c in cfgs])
@@ -282,9 +288,12 @@ class MpmSimulation:
 
     # -- materials / parameters --
     def set_materials(self, rheologies: Iterable, scene: int = 0):
+        rheologies = list(rheologies)
         rows = [material_row(r) for r in rheologies]
         arr = (Material * len(rows))(*[Material(*[float(t) for t in r]) for r in rows])
         self._check(lib().mpm_set_materials(self._h, scene, arr, len(rows)))
+        self.cfgs[scene] = self.cfgs[scene].copy(materials=rheologies)
+        self.cfg = self.cfgs[0]
 
     def set_particle_materials(self, ids):
         self.write_field("material", ids)
@@ -327,6 +336,22 @@ class MpmSimulation:
 
     def step_n(self, dt: float, substeps: int):
         self._check(lib().mpm_step_n(self._h, float(dt), int(substeps)))
+
+    # -- soil checkpoint (mpm_checkpoint.rs:388-541) --
+    def checkpoint_soil(self):
+        """``checkpoint_soil_with``: an ``MpmSoilCheckpoint`` of the particle state."""
+        from .checkpoint import checkpoint_soil
+
+        return checkpoint_soil(self)
+
+    def restore_soil(self, checkpoint):
+        """``restore_soil_with``: replace the particles with a checkpoint's."""
+        from .checkpoint import restore_soil
+
+        restore_soil(self, checkpoint)
+
+    checkpoint_soil_with = checkpoint_soil
+    restore_soil_with = restore_soil
 
     def step_n_graph(self, dt: float, substeps: int):
         self._check(lib().mpm_step_n_graph(self._h, float(dt), int(substeps)))

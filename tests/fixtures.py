@@ -80,3 +80,35 @@ def reference_key(cfg, p) -> int:
         t.append(int(c) // 4)
     w = (cfg.grid_size + 3) // 4
     return t[0] + w * (t[1] + w * t[2])
+
+
+def soil_fixture(count: int):
+    """The checkpoint fixture of mpm_checkpoint_tests.rs:3-60: MpmConfig::sand()
+    with max_particles 129, max_rigid 2, E 1000, a sand/snow/viscous material
+    table, and ``count`` particles whose nine streams follow the particle index.
+    Returns (MpmConfig, materials, words as uint32)."""
+    from molla_amd.config import Sand, Snow, Viscous
+
+    cfg = MpmConfig.sand().copy(max_particles=129, max_rigid=2, youngs=1000.0)
+    materials = [Sand(friction_angle=0.5), Snow(theta_c=0.025, theta_s=0.0075, hardening=10.0),
+                 Viscous(velocity_damping=3.0, affine_damping=5.0)]
+    vol, mass = f32(cfg.particle_volume), f32(cfg.particle_mass)
+    streams = [[] for _ in range(9)]
+    for i in range(count):
+        f = f32(i)
+        streams[0].append([f32(-0.13) + f32(i % 7) * f32(0.021), f32(0.3) + f32(i // 7 % 7) * f32(0.021),
+                           f32(-0.07) + f32(i // 49) * f32(0.021), f32(-0.0)])
+        streams[1].append([f32(0.013) + f * f32(0.0001), f32(-0.007), f32(0.002), f32(0.0)])
+        streams[2].append([0.01, 0.002, 0.0, 0.0, 0.0, -0.02, 0.003, 0.0, 0.004, 0.0, 0.008, 0.0])
+        streams[3].append([0.97, 0.001, 0.0, 0.0, 0.0, 1.01, 0.002, 0.0, 0.001, 0.0, 0.98, 0.0])
+        streams[4].append([vol * (f32(1.0) + f * f32(0.002))])
+        streams[5].append([mass * (f32(1.0) + f * f32(0.001))])
+        streams[6].append([f32(-0.1) - f * f32(0.001)])
+        streams[8].append([f32(0.005) + f * f32(0.00001), f32(0.01) + f * f32(0.001), f32(0.02), f32(3.0)])
+    parts = []
+    for s in range(9):
+        if s == 7:
+            parts.append(np.arange(count, dtype=np.uint32) % 3)
+        else:
+            parts.append(np.asarray(streams[s], np.float32).reshape(-1).view(np.uint32))
+    return cfg, materials, np.concatenate(parts).astype(np.uint32)
