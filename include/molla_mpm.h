@@ -264,7 +264,7 @@ int mpm_soil_decode_meta(const uint8_t* bytes, uint64_t len, uint32_t max_partic
  * words (words_cap >= 40 * count). */
 int mpm_soil_decode(const uint8_t* bytes, uint64_t len, uint32_t max_particles, mpm_soil_meta* meta,
                     uint32_t* words, uint64_t words_cap);
-/* MpmSoilCheckpoint::validate (:274-330): configuration ranges, rheology
+/* MpmSoilCheckpoint::validate (:281-330): configuration ranges, rheology
  * parameters, finite particle history, positive mass and volume, material
  * ids inside the table. */
 int mpm_soil_validate(const mpm_soil_meta* meta, const uint32_t* words);
@@ -279,14 +279,14 @@ int mpm_blake3(const void* data, uint64_t len, uint8_t out[32]);
 const char* mpm_soil_last_error(void);
 
 /* Device side of checkpoint_soil_with (:388-431, with encode_checkpoint_copy
- * :356-372 and CheckpointMetadata::finish :103-122): the nine streams of all
+ * :352-369 and CheckpointMetadata::finish :111-133): the nine streams of all
  * `n` particles (n = the particle count) in slot order, one gather kernel and
  * one device-to-host copy.  Single-scene MPM_MODEL_SHADER handles only.
  * Fails with MPM_ERR_FAULT while a grid fault is latched, and after
  * mpm_reset_status() of a latched fault until a step or restore has run
  * (an incomplete state cannot be archived). */
 int mpm_checkpoint_capture(mpm_sim* sim, uint32_t* words, uint32_t n);
-/* Device side of restore_soil_with (:439-541: validate_soil_restore_origin,
+/* Device side of restore_soil_with (:436-541: validate_soil_restore_origin,
  * encode_soil_restore and finish_soil_restore).  `meta` and `words` are a
  * decoded checkpoint; `rows` is its material table packed as material_row
  * (mpm.rs:666-686) does, n_rows = max(meta->n_materials, 1).  The call

@@ -175,7 +175,7 @@ class MpmSoilCheckpoint:
         return w
 
     def validate(self):
-        """``validate`` (:274-330)."""
+        """``validate`` (:281-330)."""
         w = self._words()
         _codec_check(_lib().mpm_soil_validate(C.byref(self.meta()), w.ctypes.data_as(C.c_void_p)))
 
@@ -247,7 +247,7 @@ def _soil_config(cfg: MpmConfig) -> MpmConfig:
 
 
 def restore_soil(sim, ckpt: MpmSoilCheckpoint):
-    """``restore_soil_with`` (:439-541): replace the particle set, material
+    """``restore_soil_with`` (:436-541): replace the particle set, material
     table and grid window; nothing changes when the checkpoint is rejected."""
     from .mpm import MpmError
 
@@ -255,7 +255,7 @@ def restore_soil(sim, ckpt: MpmSoilCheckpoint):
         raise MpmError(1, "the soil checkpoint is defined for single-scene shader-model simulations")
     w = ckpt._words()
     m = ckpt.meta()
-    # validate_soil_restore_origin (:457-487): every configuration word but the
+    # validate_soil_restore_origin (:457-490): every configuration word but the
     # capacities must match this simulation's
     mine = config_words(sim.cfg)
     theirs = config_words(ckpt.cfg.copy(max_particles=sim.cfg.max_particles, max_rigid=sim.cfg.max_rigid))

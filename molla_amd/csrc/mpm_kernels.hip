@@ -7,7 +7,8 @@
 //                  p2g/p2g_two_pass :494-925) so a particle's state is read
 //                  once and written once per substep and the polar/SVD of F is
 //                  computed once per substep (FP64).
-// The P2G half reduces the block's scatter in LDS (64-bit fixed point) over
+// The P2G half reduces the block's scatter in LDS (fixed point: 32-bit for the
+// shader model, 64-bit for the differentiable model) over
 // the bounding box of its particles' stencils -- particles are kept sorted by
 // 4^3 tile and cut into chunks that stay inside four tiles of one tile row,
 // so that box is small -- and stores the box as the chunk's partial with
@@ -798,7 +799,7 @@ hipError_t launch_permute_in(float* ps, uint32_t cap, int comp0, int ncomp,
     return hipGetLastError();
 }
 
-// ---- soil checkpoint streams (mpm_checkpoint.rs:78-83, :356-372) ----------
+// ---- soil checkpoint streams (mpm_checkpoint.rs:78-83, :352-369) ----------
 // The archive holds n particles as nine streams, stream-major, 40 words per
 // particle: position+w (4), velocity+w (4), C (12), F (12), volume, mass,
 // log-Jp, material id, tire compaction (4).  C and F are three 16-byte

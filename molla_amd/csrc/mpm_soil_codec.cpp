@@ -2,7 +2,7 @@
 //
 // Byte format of MpmSoilCheckpoint (crates/molla-solvers/src/mpm_checkpoint.rs):
 // header :78-85, digest :87-94, to_bytes :185-214, from_bytes :216-267,
-// validate :274-330, rheology_words/validate_rheology :544-592,
+// validate :281-330, rheology_words/validate_rheology :544-592,
 // config_words :594-619, Words reader :621-700; the lattice window check is
 // validate_lattice_window (mpm_soil_lattice_window.rs:6-39) with
 // validate_node_window (soil_candidates_gpu.rs:68-94).  The format is shared

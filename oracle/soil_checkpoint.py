@@ -9,7 +9,7 @@ checked byte for byte:
   digest           :87-94   (BLAKE3 of [version bytes, v2 only] || payload)
   encode           :185-214 (to_bytes)
   decode           :216-267 (from_bytes, Words reader :621-700)
-  validate         :274-330 (with validate_rheology :559-592)
+  validate         :281-330 (with validate_rheology :559-592)
 
 A checkpoint here is a dict: window (3 ints), config (21 u32), materials
 (list of 4-word rheology encodings), count, words (list or array of u32).
