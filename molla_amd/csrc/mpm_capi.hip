@@ -124,6 +124,15 @@ struct mpm_sim {
     float4* gadj = nullptr;    // per-cell grid adjoint
     double* gml = nullptr;     // per-chunk (dJ/dmu, dJ/dlambda)
     bool adj_valid = false;
+    // device words of the differentiable path: [0] saved gather count, [4, 8)
+    // the summed (dJ/dmu, dJ/dlambda), then the escape/fallback counters at
+    // the forward's segment boundaries (2 per boundary) and around each
+    // segment's recompute in the backward (4 per segment)
+    uint32_t* dsnap = nullptr;
+    uint32_t dsnap_len = 0;
+    std::vector<uint8_t> seg_fused;  // per segment: the forward saw no escape or fallback scatter
+    void* hpin = nullptr;            // pinned host staging of the differentiable path
+    size_t hpin_bytes = 0;
     // soil checkpoint: the archived words the planes do not hold (position and
     // velocity w, C/F column pads, tire compaction; 12 per particle, by id)
     uint32_t* ck_aux = nullptr;
@@ -222,9 +231,10 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux};
+    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap};
     for (void* b : dbufs)
         if (b) hipFree(b);
+    if (s->hpin) hipHostFree(s->hpin);
     if (s->stream) hipStreamDestroy(s->stream);
 }
 
@@ -728,6 +738,16 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
         HIP_TRY(s, dalloc(&c, ck));
         s->ckpt.push_back(c);
     }
+    const uint32_t snap_words = 8 + 2 * (nseg + 1) + 4 * nseg;
+    if (s->dsnap_len < snap_words) {
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        if (s->dsnap) HIP_TRY(s, hipFree(s->dsnap));
+        s->dsnap = nullptr;
+        s->dsnap_len = 0;
+        HIP_TRY(s, dalloc(&s->dsnap, snap_words));
+        s->dsnap_len = snap_words;
+    }
+    uint32_t* fsnap = s->dsnap + 8;
     const int mode = s->p2g_mode, wvc = s->write_vc;
     s->p2g_mode = MPM_P2G_FUSED;
     s->write_vc = 1;
@@ -736,17 +756,30 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
     rc = clear_grid(s);
     if (rc == MPM_OK) rc = sort_particles(s, true);
     s->hold_order = true;
+    // The escape and fallback counters at every segment boundary tell the
+    // backward which segments it may reverse with the fused kernel, so it
+    // needs no host round trip per segment (the recompute repeats them).
+    hipError_t e = hipSuccess;
+    if (rc == MPM_OK && (e = mpm::launch_snap_counters(s->status, fsnap, s->stream)) != hipSuccess)
+        rc = hip_fail(s, e, "counter snapshot");
     for (uint32_t seg = 0; rc == MPM_OK && seg < nseg; ++seg) {
         const uint32_t len = std::min(every, steps - seg * every);
-        hipError_t e = hipMemcpyAsync(s->ckpt[seg], s->ps[s->ps_cur], ck * 4, hipMemcpyDeviceToDevice, s->stream);
+        e = hipMemcpyAsync(s->ckpt[seg], s->ps[s->ps_cur], ck * 4, hipMemcpyDeviceToDevice, s->stream);
         if (e != hipSuccess) rc = hip_fail(s, e, "checkpoint copy");
         if (rc == MPM_OK) rc = run_step_n(s, dt, len);
+        if (rc == MPM_OK && (e = mpm::launch_snap_counters(s->status, fsnap + 2 * (seg + 1), s->stream)) != hipSuccess)
+            rc = hip_fail(s, e, "counter snapshot");
     }
     s->hold_order = false;
     s->p2g_mode = mode;
     s->write_vc = wvc;
     if (rc != MPM_OK) return rc;
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> snap(2 * (nseg + 1));
+    HIP_TRY(s, hipMemcpy(snap.data(), fsnap, snap.size() * 4, hipMemcpyDeviceToHost));
+    s->seg_fused.assign(nseg, 0);
+    for (uint32_t seg = 0; seg < nseg; ++seg)
+        s->seg_fused[seg] = snap[2 * seg] == snap[2 * seg + 2] && snap[2 * seg + 1] == snap[2 * seg + 3];
     s->diff_steps = steps;
     s->diff_every = every;
     s->diff_dt = dt;
@@ -768,30 +801,28 @@ mpm::AdjArgs adj_args(mpm_sim* s, float dt, int list, float* ps, int& blocks) {
     return q;
 }
 
-int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
-    if (s->diff_steps == 0) return fail(s, MPM_ERR_INVALID, "mpm_diff_forward must run first");
-    if (s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
-        return fail(s, MPM_ERR_INVALID, "the particle order changed since mpm_diff_forward");
-    if (!gx) return fail(s, MPM_ERR_INVALID, "grad_x_final must not be NULL");
+int ensure_pinned(mpm_sim* s, size_t bytes) {
+    if (s->hpin_bytes >= bytes) return MPM_OK;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (s->hpin) HIP_TRY(s, hipHostFree(s->hpin));
+    s->hpin = nullptr;
+    s->hpin_bytes = 0;
+    HIP_TRY(s, hipHostMalloc(&s->hpin, bytes, hipHostMallocDefault));
+    s->hpin_bytes = bytes;
+    return MPM_OK;
+}
+
+constexpr int BACKWARD_REPLAN = -1;  // a segment's recompute escaped where its forward did not
+
+// One reverse sweep.  `checked`: read the escape counters around every
+// segment's recompute on the host (two synchronisations per segment) instead
+// of using the plan the forward recorded.
+int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checked) {
     const uint32_t cap = s->cap, every = s->diff_every, steps = s->diff_steps;
     const size_t plane = cap;
     const size_t cells = (size_t)s->total_tiles * 64;
-    if (s->tape_len < every) {
-        if (s->tape) HIP_TRY(s, hipFree(s->tape));
-        if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
-        s->tape = nullptr;
-        s->gtape = nullptr;
-        s->tape_len = 0;
-        HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * plane));
-        HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
-        s->tape_len = every;
-    }
-    if (!s->adj) HIP_TRY(s, dalloc(&s->adj, (size_t)mpm::NADJ * plane));
-    if (!s->gml) HIP_TRY(s, dalloc(&s->gml, 2 * (size_t)s->max_schunks));
-    if (!s->gadj) {
-        HIP_TRY(s, dalloc(&s->gadj, cells));
-        HIP_TRY(s, hipMemsetAsync(s->gadj, 0, cells * sizeof(float4), s->stream));
-    }
+    const uint32_t nseg = (steps + every - 1) / every;
+    uint32_t* bsnap = s->dsnap + 8 + 2 * (nseg + 1);
     const int mode = s->p2g_mode;
     s->p2g_mode = MPM_P2G_FUSED;
     int rc = clear_grid(s);
@@ -799,23 +830,24 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
         s->p2g_mode = mode;
         return rc;
     }
-    uint32_t gathers = 0;
-    HIP_TRY(s, hipMemcpyAsync(&gathers, s->status + mpm::ST_GATHERS, 4, hipMemcpyDeviceToHost, s->stream));
+    // the recompute's G2P passes must not count as completed gathers
+    HIP_TRY(s, mpm::launch_copy_u32(s->status + mpm::ST_GATHERS, s->dsnap, s->stream));
     HIP_TRY(s, hipMemsetAsync(s->adj, 0, (size_t)mpm::NADJ * plane * 4, s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gml, 0, 2 * (size_t)s->max_schunks * sizeof(double), s->stream));
-    HIP_TRY(s, hipMemcpyAsync(s->staging, gx, (size_t)s->n * 12, hipMemcpyHostToDevice, s->stream));
+    const size_t vec = (size_t)s->n * 12;
+    std::memcpy(s->hpin, gx, vec);
+    if (gv) std::memcpy(static_cast<char*>(s->hpin) + vec, gv, vec);
+    HIP_TRY(s, hipMemcpyAsync(s->staging, s->hpin, gv ? 2 * vec : vec, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AX, 3, s->slot_of_id, s->staging, s->n, s->stream));
-    if (gv) {
-        HIP_TRY(s, hipMemcpyAsync(s->staging, gv, (size_t)s->n * 12, hipMemcpyHostToDevice, s->stream));
-        HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AV, 3, s->slot_of_id, s->staging, s->n, s->stream));
-    }
+    if (gv)
+        HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AV, 3, s->slot_of_id, s->staging + 3 * (size_t)s->n,
+                                          s->n, s->stream));
     const float dt = s->diff_dt;
     const size_t st_words = (size_t)mpm::NCOMP * plane;
     const mpm::GridDev g = griddev(s);
     const int gb = grid_blocks(s);
     int L = s->list_cur;
     auto count = [&](int l) { return s->status + mpm::ST_COUNT0 + l; };
-    const uint32_t nseg = (steps + every - 1) / every;
     // planes 24.. (jp, mass, volume, E, material, id) do not change in the
     // CPU-diff model: every tape slot gets them once
     for (uint32_t k = 0; k < every; ++k)
@@ -823,10 +855,16 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
                                   (size_t)(mpm::NCOMP - 24) * plane * 4, hipMemcpyDeviceToDevice, s->stream));
     for (int seg = int(nseg) - 1; seg >= 0 && rc == MPM_OK; --seg) {
         const uint32_t len = std::min(every, steps - uint32_t(seg) * every);
-        uint32_t w0[mpm::ST_WORDS];
-        HIP_TRY(s, hipMemcpyAsync(w0, s->status, sizeof(w0), hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(s, hipStreamSynchronize(s->stream));
-        const uint32_t esc0 = w0[mpm::ST_ESCAPES], fb0 = w0[mpm::ST_FALLBACK];
+        uint32_t esc0 = 0, fb0 = 0;
+        if (checked) {
+            uint32_t w0[mpm::ST_WORDS];
+            HIP_TRY(s, hipMemcpyAsync(w0, s->status, sizeof(w0), hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(s, hipStreamSynchronize(s->stream));
+            esc0 = w0[mpm::ST_ESCAPES];
+            fb0 = w0[mpm::ST_FALLBACK];
+        } else {
+            HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg, s->stream));
+        }
         float* t0 = s->tape;
         HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
         // Recompute the segment with the fused forward: substep k's grid goes
@@ -857,10 +895,16 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
         // No particle of the segment scattered with global atomics (its reverse
         // scatters use the same stencils): the reverse P2G of substep k and the
         // reverse G2P of substep k-1 run as one pass.
-        uint32_t w1[mpm::ST_WORDS];
-        HIP_TRY(s, hipMemcpyAsync(w1, s->status, sizeof(w1), hipMemcpyDeviceToHost, s->stream));
-        HIP_TRY(s, hipStreamSynchronize(s->stream));
-        const bool fused = w1[mpm::ST_ESCAPES] == esc0 && w1[mpm::ST_FALLBACK] == fb0;
+        bool fused;
+        if (checked) {
+            uint32_t w1[mpm::ST_WORDS];
+            HIP_TRY(s, hipMemcpyAsync(w1, s->status, sizeof(w1), hipMemcpyDeviceToHost, s->stream));
+            HIP_TRY(s, hipStreamSynchronize(s->stream));
+            fused = w1[mpm::ST_ESCAPES] == esc0 && w1[mpm::ST_FALLBACK] == fb0;
+        } else {
+            HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg + 2, s->stream));
+            fused = s->seg_fused[seg] != 0;
+        }
         auto tape_at = [&](int k) { return s->tape + (size_t)k * st_words; };
         auto gv_at = [&](int k) { return s->gtape + (size_t)2 * k * cells; };
         if (fused) {
@@ -906,8 +950,53 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     s->grid_dirty = false;
     s->p2g_mode = mode;
     if (rc != MPM_OK) return rc;
-    HIP_TRY(s, hipMemcpyAsync(s->status + mpm::ST_GATHERS, &gathers, 4, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(s, mpm::launch_copy_u32(s->dsnap, s->status + mpm::ST_GATHERS, s->stream));
+    if (!checked)
+        HIP_TRY(s, hipMemcpyAsync(s->hpin, bsnap, (size_t)16 * nseg, hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (!checked) {
+        const uint32_t* w = static_cast<const uint32_t*>(s->hpin);
+        for (uint32_t seg = 0; seg < nseg; ++seg)
+            if (s->seg_fused[seg] && (w[4 * seg] != w[4 * seg + 2] || w[4 * seg + 1] != w[4 * seg + 3]))
+                return BACKWARD_REPLAN;
+    }
+    return MPM_OK;
+}
+
+int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
+    if (s->diff_steps == 0) return fail(s, MPM_ERR_INVALID, "mpm_diff_forward must run first");
+    if (s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
+        return fail(s, MPM_ERR_INVALID, "the particle order changed since mpm_diff_forward");
+    if (!gx) return fail(s, MPM_ERR_INVALID, "grad_x_final must not be NULL");
+    const uint32_t every = s->diff_every, steps = s->diff_steps;
+    const uint32_t nseg = (steps + every - 1) / every;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    if (s->tape_len < every) {
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        if (s->tape) HIP_TRY(s, hipFree(s->tape));
+        if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
+        s->tape = nullptr;
+        s->gtape = nullptr;
+        s->tape_len = 0;
+        HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * s->cap));
+        HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
+        s->tape_len = every;
+    }
+    if (!s->adj) HIP_TRY(s, dalloc(&s->adj, (size_t)mpm::NADJ * s->cap));
+    if (!s->gml) HIP_TRY(s, dalloc(&s->gml, 2 * (size_t)s->max_schunks));
+    if (!s->gadj) {
+        HIP_TRY(s, dalloc(&s->gadj, cells));
+        HIP_TRY(s, hipMemsetAsync(s->gadj, 0, cells * sizeof(float4), s->stream));
+    }
+    // pinned staging: the final-state gradients in, the initial-state ones
+    // (x, v, E and the (mu, lambda) pair) out, the counter snapshots
+    const size_t pin = std::max<size_t>({(size_t)s->n * 24, (size_t)s->n * 28 + 64, (size_t)16 * nseg});
+    int rc = ensure_pinned(s, pin);
+    if (rc != MPM_OK) return rc;
+    s->adj_valid = false;
+    rc = diff_backward_pass(s, gx, gv, false);
+    if (rc == BACKWARD_REPLAN) rc = diff_backward_pass(s, gx, gv, true);
+    if (rc != MPM_OK) return rc;
     s->adj_valid = true;
     return MPM_OK;
 }
@@ -932,19 +1021,37 @@ int read_adjoint(mpm_sim* s, int field, float* out) {
     return MPM_OK;
 }
 
-int read_mu_lambda(mpm_sim* s, double* out) {
-    uint32_t nch = 0;
-    HIP_TRY(s, hipMemcpy(&nch, s->status + mpm::ST_NCHUNKS, 4, hipMemcpyDeviceToHost));
-    nch = std::min(nch, s->max_schunks);
-    std::vector<double> h(2 * (size_t)nch);
-    if (nch) HIP_TRY(s, hipMemcpy(h.data(), s->gml, h.size() * sizeof(double), hipMemcpyDeviceToHost));
-    double m = 0.0, l = 0.0;
-    for (uint32_t i = 0; i < nch; ++i) {
-        m += h[2 * i];
-        l += h[2 * i + 1];
+// The outputs of a backward in one transfer: the requested adjoint planes in
+// original order, then the fixed-order sum of the per-chunk (mu, lambda).
+int read_backward_outputs(mpm_sim* s, float* gx0, float* gv0, float* gE, double* gml) {
+    const size_t n = s->n;
+    struct Out {
+        float* dst;
+        int comp, nc;
+    } outs[3] = {{gx0, mpm::AX, 3}, {gv0, mpm::AV, 3}, {gE, mpm::AE, 1}};
+    size_t off = 0;
+    for (const Out& o : outs) {
+        if (!o.dst) continue;
+        HIP_TRY(s, mpm::launch_unpermute(s->adj, s->cap, o.comp, o.nc, s->slot_of_id, s->staging + off, s->n,
+                                         s->stream));
+        off += n * o.nc;
     }
-    out[0] = m;
-    out[1] = l;
+    const size_t dbl = (off * 4 + 15) / 16 * 16;  // byte offset of the pair in the pinned buffer
+    char* pin = static_cast<char*>(s->hpin);
+    if (off) HIP_TRY(s, hipMemcpyAsync(pin, s->staging, off * 4, hipMemcpyDeviceToHost, s->stream));
+    if (gml) {
+        double* pair = reinterpret_cast<double*>(s->dsnap + 4);
+        HIP_TRY(s, mpm::launch_sum_pairs(s->gml, s->status + mpm::ST_NCHUNKS, s->max_schunks, pair, s->stream));
+        HIP_TRY(s, hipMemcpyAsync(pin + dbl, pair, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    }
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    off = 0;
+    for (const Out& o : outs) {
+        if (!o.dst) continue;
+        std::memcpy(o.dst, pin + off * 4, n * o.nc * 4);
+        off += n * o.nc;
+    }
+    if (gml) std::memcpy(gml, pin + dbl, 2 * sizeof(double));
     return MPM_OK;
 }
 
@@ -1365,11 +1472,7 @@ int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad
     if (!sim) return MPM_ERR_INVALID;
     int rc = diff_backward(sim, grad_x_final, grad_v_final);
     if (rc != MPM_OK) return rc;
-    if (grad_x0 && (rc = read_adjoint(sim, MPM_FIELD_POSITION, grad_x0)) != MPM_OK) return rc;
-    if (grad_v0 && (rc = read_adjoint(sim, MPM_FIELD_VELOCITY, grad_v0)) != MPM_OK) return rc;
-    if (grad_E && (rc = read_adjoint(sim, MPM_FIELD_YOUNGS, grad_E)) != MPM_OK) return rc;
-    if (grad_mu_lambda && (rc = read_mu_lambda(sim, grad_mu_lambda)) != MPM_OK) return rc;
-    return MPM_OK;
+    return read_backward_outputs(sim, grad_x0, grad_v0, grad_E, grad_mu_lambda);
 }
 
 int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,

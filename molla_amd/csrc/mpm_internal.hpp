@@ -152,6 +152,9 @@ hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, floa
                                   const uint32_t* dlist, const uint32_t* dcount, float dt, int blocks,
                                   hipStream_t s);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+hipError_t launch_snap_counters(const uint32_t* status, uint32_t* dst, hipStream_t s);
+hipError_t launch_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s);
+hipError_t launch_sum_pairs(const double* v, const uint32_t* count, uint32_t cap, double* out, hipStream_t s);
 // Static active list of a sort: the 3x3x3 dilation of occupied tiles, in
 // ascending tile order (mark/pos: total_tiles u32 each; scan_scratch:
 // scan_scratch_words(total_tiles)).  Also builds the tile lower-bound table.

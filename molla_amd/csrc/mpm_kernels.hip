@@ -600,6 +600,51 @@ hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Status bookkeeping of the differentiable path without host round trips:
+// the escape and fallback counters into dst[0..2).
+__global__ void k_snap_counters(const uint32_t* status, uint32_t* dst) {
+    dst[0] = status[ST_ESCAPES];
+    dst[1] = status[ST_FALLBACK];
+}
+
+hipError_t launch_snap_counters(const uint32_t* status, uint32_t* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_snap_counters, dim3(1), dim3(1), 0, s, status, dst);
+    return hipGetLastError();
+}
+
+// Sum of the per-chunk (dJ/dmu, dJ/dlambda) pairs v[2i], v[2i+1], i < *count
+// (clamped to cap), in a fixed order: thread t adds entries t, t+256, ...
+// in sequence, then a fixed tree combines the 256 partials.  One block.
+__global__ __launch_bounds__(256) void k_sum_pairs(const double* __restrict__ v, const uint32_t* count, uint32_t cap,
+                                                   double* __restrict__ out) {
+    __shared__ double sm[2][256];
+    const uint32_t n = min(*count, cap);
+    double a = 0.0, b = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        a += v[2 * (size_t)i];
+        b += v[2 * (size_t)i + 1];
+    }
+    sm[0][threadIdx.x] = a;
+    sm[1][threadIdx.x] = b;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + h];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sm[0][0];
+        out[1] = sm[1][0];
+    }
+}
+
+hipError_t launch_sum_pairs(const double* v, const uint32_t* count, uint32_t cap, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(256), 0, s, v, count, cap, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also) {
     hipLaunchKernelGGL(k_clear_tiles, dim3(blocks), dim3(256), 0, s, g, dlist, dcount, also);

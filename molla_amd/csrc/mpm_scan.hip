@@ -163,6 +163,12 @@ __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t
 
 __global__ void k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst) { *dst = *src; }
 
+// One device word to another, on the stream (no host round trip).
+hipError_t launch_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, s, src, dst);
+    return hipGetLastError();
+}
+
 hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
                                const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
