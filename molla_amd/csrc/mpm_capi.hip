@@ -90,6 +90,7 @@ struct mpm_sim {
     int list_cur = 0;
     bool grid_dirty = false;
     bool sorted = false;
+    bool boxes_valid = false;  // chunk_box holds the P2G boxes of the current positions
     uint32_t since_sort = 0;
     uint32_t sorts = 0;
     int has_E = 0;
@@ -262,6 +263,7 @@ mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
     a.use_lds = s->p2g_mode == MPM_P2G_FUSED ? 1 : 0;
     a.write_vc = s->write_vc;
     a.has_E = s->has_E;
+    a.box_gather = s->boxes_valid ? 1 : 0;
     return a;
 }
 
@@ -384,6 +386,7 @@ __global__ void k_iota(uint32_t* p, uint32_t n) {
 
 // keys[id] -> stable sort -> reorder particle planes into tile order.
 int sort_particles(mpm_sim* s, bool reorder) {
+    s->boxes_valid = false;  // the chunk list is rebuilt
     if (s->dry) {
         if (reorder) {
             s->ps_cur = 1 - s->ps_cur;
@@ -439,12 +442,21 @@ void use_sorted_chunks(mpm_sim* s, mpm::StepArgs& a, int& blocks) {
 // `ps` (optional) runs the pass on another set of state planes of the same
 // slot order (the adjoint's recomputed pre-states).
 int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = nullptr) {
-    if (s->dry) return MPM_OK;
+    // a scattering pass over the sorted chunks leaves their boxes for the
+    // next pass's gather; a plain G2P moves the particles away from them
+    const bool sorted_chunks = s->p2g_mode == MPM_P2G_FUSED && s->sorted;
+    const bool boxes_after = sorted_chunks && mode != mpm::MODE_G2P;
+    if (s->dry) {
+        s->boxes_valid = boxes_after;
+        return MPM_OK;
+    }
     Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
     mpm::StepArgs a = args(s, dt, list_out);
     if (ps) a.ps = a.ps_out = ps;
     int blocks = particle_blocks(s);
-    if (s->p2g_mode == MPM_P2G_FUSED && s->sorted) use_sorted_chunks(s, a, blocks);
+    if (sorted_chunks) use_sorted_chunks(s, a, blocks);
+    else a.box_gather = 0;
+    s->boxes_valid = boxes_after;
     HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, blocks, s->stream));
     return MPM_OK;
 }
@@ -949,6 +961,7 @@ int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checke
     s->list_cur = L;
     s->grid_dirty = false;
     s->p2g_mode = mode;
+    s->boxes_valid = false;  // the reverse passes left their own boxes
     if (rc != MPM_OK) return rc;
     HIP_TRY(s, mpm::launch_copy_u32(s->dsnap, s->status + mpm::ST_GATHERS, s->stream));
     if (!checked)

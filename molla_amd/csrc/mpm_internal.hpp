@@ -133,6 +133,8 @@ struct StepArgs {
     int use_lds;   // 0: every node goes straight to global atomics
     int write_vc;  // 1: store v and C every substep
     int has_E;     // per-particle Young's modulus plane is valid
+    int box_gather;  // 1: chunk_box holds, per chunk of this pass, the P2G box of
+                     // the current positions (the previous pass scattered them)
 };
 
 // ---- launchers (mpm_kernels.hip) ----

@@ -81,6 +81,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
         const bool active = threadIdx.x < ch.count;
+        // Gather staging: the chunk's P2G box of the previous pass holds every
+        // stencil this gather reads (same particles, same positions), so the
+        // block loads that box of grid velocities into LDS once and the 27
+        // node reads of each particle come from LDS.  Particles outside the
+        // box (escaped ones) read global memory.
+        int4 gbox = make_int4(0, 0, 0, 0), gdim = make_int4(0, 0, 0, 0);
+        const bool staged = MODE != MODE_P2G && stage_gather_box(a, sc, ci, L, gbox, gdim);
         Scatter sp;
         sp.on = false;
         sp.esc = false;
@@ -155,41 +162,57 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                     float M[9];
 #pragma unroll
                     for (int k = 0; k < 9; ++k) M[k] = 0.f;
-                    const int G = int(sc.G);
+                    // fetch(i, j, k, g): node (b + (i, j, k)), false when outside the grid
+                    auto gather = [&](auto fetch) {
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int iz = b[2] + k;
-                        if (iz < 0 || iz >= G) continue;
+                        for (int k = 0; k < 3; ++k) {
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) {
-                            const int iy = b[1] + j;
-                            if (iy < 0 || iy >= G) continue;
-                            const float wjk = w[1][j] * w[2][k];
-                            float rs[3] = {0.f, 0.f, 0.f}, ri[3] = {0.f, 0.f, 0.f};
+                            for (int j = 0; j < 3; ++j) {
+                                const float wjk = w[1][j] * w[2][k];
+                                float rs[3] = {0.f, 0.f, 0.f}, ri[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-                            for (int i = 0; i < 3; ++i) {
-                                const int ix = b[0] + i;
-                                if (ix < 0 || ix >= G) continue;
-                                const float4 g = a.gv[cell_index(sc, ix, iy, iz)];
-                                const float wt = w[0][i] * wjk;
-                                const float gx = wt * g.x, gy = wt * g.y, gz = wt * g.z;
-                                rs[0] += gx;
-                                rs[1] += gy;
-                                rs[2] += gz;
-                                if (i > 0) {
-                                    ri[0] += float(i) * gx;
-                                    ri[1] += float(i) * gy;
-                                    ri[2] += float(i) * gz;
+                                for (int i = 0; i < 3; ++i) {
+                                    float4 g;
+                                    if (!fetch(i, j, k, g)) continue;
+                                    const float wt = w[0][i] * wjk;
+                                    const float gx = wt * g.x, gy = wt * g.y, gz = wt * g.z;
+                                    rs[0] += gx;
+                                    rs[1] += gy;
+                                    rs[2] += gz;
+                                    if (i > 0) {
+                                        ri[0] += float(i) * gx;
+                                        ri[1] += float(i) * gy;
+                                        ri[2] += float(i) * gz;
+                                    }
+                                }
+#pragma unroll
+                                for (int r = 0; r < 3; ++r) {
+                                    vn[r] += rs[r];
+                                    M[3 * r] += ri[r];
+                                    M[3 * r + 1] += float(j) * rs[r];
+                                    M[3 * r + 2] += float(k) * rs[r];
                                 }
                             }
-#pragma unroll
-                            for (int r = 0; r < 3; ++r) {
-                                vn[r] += rs[r];
-                                M[3 * r] += ri[r];
-                                M[3 * r + 1] += float(j) * rs[r];
-                                M[3 * r + 2] += float(k) * rs[r];
-                            }
                         }
+                    };
+                    const bool in_box = staged && b[0] >= gbox.x && b[1] >= gbox.y && b[2] >= gbox.z &&
+                                        b[0] + 2 < gbox.x + gdim.x && b[1] + 2 < gbox.y + gdim.y &&
+                                        b[2] + 2 < gbox.z + gdim.z;
+                    if (in_box) {
+                        const float4* stage = reinterpret_cast<const float4*>(&L.acc[0][0]);
+                        const int li0 = (b[0] - gbox.x) + gdim.x * ((b[1] - gbox.y) + gdim.y * (b[2] - gbox.z));
+                        gather([&](int i, int j, int k, float4& g) {
+                            g = stage[li0 + i + gdim.x * (j + gdim.y * k)];
+                            return true;
+                        });
+                    } else {
+                        const int G = int(sc.G);
+                        gather([&](int i, int j, int k, float4& g) {
+                            const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
+                            if (ix < 0 || ix >= G || iy < 0 || iy >= G || iz < 0 || iz >= G) return false;
+                            g = a.gv[cell_index(sc, ix, iy, iz)];
+                            return true;
+                        });
                     }
                     const float dx = sc.dx, s4dx = 4.0f * sc.inv_dx;
                     float Cn[9];
@@ -374,6 +397,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
             }
         }
         if (MODE != MODE_G2P) block_scatter<true, WIDE>(a, sc, sp, ci, L);
+        else if (staged) __syncthreads();  // the next chunk's staging overwrites the box
     }
 }
 

@@ -102,7 +102,9 @@ struct Scatter {
 //     adjoint's dJ/d grid v).
 template <bool WIDE>
 struct ScatterLdsT {
-    typename std::conditional<WIDE, unsigned long long, int>::type acc[4][LDS_CELLS];  // (m, px, py, pz)
+    // (m, px, py, pz); before the scatter the same space stages the gather's
+    // grid velocities (LDS_CELLS float4)
+    alignas(16) typename std::conditional<WIDE, unsigned long long, int>::type acc[4][LDS_CELLS];
     int red[WAVES * 6];
     float redf[WAVES * 2];
     int box[16];
@@ -354,6 +356,33 @@ __device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDe
     for (int k = 0; k < 3; ++k)
         if (b[k] < 4 * tk[k] - 4 || b[k] > 4 * tk[k] + 5) esc = true;
     return esc;
+}
+
+// Loads the grid velocities of chunk `ci`'s last P2G box into L.acc (as
+// float4, box-local index lx + bx (ly + by lz)) when that box describes this
+// pass's stencils: the previous pass scattered the same chunk list from the
+// current positions (a.box_gather) and reduced it in LDS (box extent > 0).
+// Block-uniform; every thread calls it.  Returns whether the box is staged
+// (origin in o, extent in d); the caller must not overwrite L.acc before
+// every thread has finished its gather.
+template <bool WIDE>
+__device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneDev& sc, uint32_t ci,
+                                                 ScatterLdsT<WIDE>& L, int4& o, int4& d) {
+    if (!a.box_gather || !a.use_lds || ci >= a.part_capacity) return false;
+    o = a.chunk_box[2 * (size_t)ci];
+    d = a.chunk_box[2 * (size_t)ci + 1];
+    if (d.x <= 0 || d.y <= 0 || d.z <= 0 || d.x * d.y * d.z > LDS_CELLS) return false;
+    float4* stage = reinterpret_cast<float4*>(&L.acc[0][0]);
+    const int rows = d.y * d.z;
+    const float inv_by = 1.0f / float(d.y);
+    // 16 lanes along x, 16 rows (y, z) per pass
+    for (int r = int(threadIdx.x >> 4); r < rows; r += BLOCK / 16) {
+        const int lz = int((float(r) + 0.5f) * inv_by), ly = r - lz * d.y;
+        for (int lx = int(threadIdx.x & 15); lx < d.x; lx += 16)
+            stage[lx + d.x * r] = a.gv[cell_index(sc, o.x + lx, o.y + ly, o.z + lz)];
+    }
+    __syncthreads();
+    return true;
 }
 
 // Adds to q the P2G partials of every chunk that may have touched cell
