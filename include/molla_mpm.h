@@ -218,6 +218,38 @@ int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad
  * VELOCITY, C, F (row-major 3x3) or YOUNGS. */
 int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n);
 
+/* ---- rigid bodies on the grid: set_rigid_spheres (mpm.rs:1521-1566),
+ *      set_prescribed_solids (mpm_prescribed_solids.rs:91-140),
+ *      read_rigid_wrenches_with (mpm.rs:2908-2955); shader model, one scene ----
+ * Every substep, after the P2G, the first body containing a grid node gives
+ * the node its velocity (rigid_imprint, mpm.wgsl:937-1006) and the node skips
+ * gravity, walls and the implicit sweeps.  Legacy spheres stamp a heavy cell
+ * (mass 1e6) and also feel the sand in a one-cell shell outside
+ * (mpm_rigid_force_integrate, mpm.wgsl:1174-1226); prescribed solids keep
+ * the sand mass and need explicit stepping with dt > 0. */
+#define MPM_MAX_RIGID 64 /* bodies per handle; the caller's max_rigid may be lower */
+typedef enum { MPM_SOLID_SPHERE = 1, MPM_SOLID_BOX = 2, MPM_SOLID_CYLINDER = 3 } mpm_solid_shape;
+/* MpmPrescribedSolid (mpm_prescribed_solids.rs:21-32). */
+typedef struct {
+    uint32_t shape;            /* mpm_solid_shape                                       */
+    float dims[3];             /* sphere: radius; box: half extents; cylinder: radius,
+                                  half width along local y                              */
+    float center[3];
+    float orientation[4];      /* unit quaternion, xyzw                                 */
+    float linear_velocity[3];
+    float angular_velocity[3]; /* rad/s: a node moves with v + w x (node - center)      */
+} mpm_prescribed_solid;
+/* Replaces the bodies with n legacy spheres: 7 floats each (center xyz,
+ * radius, velocity xyz). */
+int mpm_set_rigid_spheres(mpm_sim* sim, const float* spheres, uint32_t n);
+/* Replaces the bodies with n prescribed solids (validated first: finite pose
+ * and velocities, unit quaternion to 1e-4, positive finite dimensions;
+ * nothing changes on failure). */
+int mpm_set_prescribed_solids(mpm_sim* sim, const mpm_prescribed_solid* solids, uint32_t n);
+/* Reaction of the last substep stepped: n = the body count, 6 floats each
+ * (force xyz, torque xyz about the body center), bodies in set order. */
+int mpm_read_rigid_wrenches(mpm_sim* sim, float* out, uint32_t n);
+
 /* ---- soil particle checkpoint, "MOLLSOIL" v1/v2 (mpm_checkpoint.rs) ----
  * Byte layout (little-endian, MpmSoilCheckpoint::to_bytes, :185-214):
  *   [0, 8)   "MOLLSOIL"                [8, 12)  version: 1, or 2 when the

@@ -18,6 +18,7 @@
 
 #include "../../include/molla_mpm.h"
 #include "mpm_internal.hpp"
+#include "mpm_rigid.hpp"
 
 using mpm::Chunk;
 using mpm::SceneDev;
@@ -134,6 +135,11 @@ struct mpm_sim {
     std::vector<uint8_t> seg_fused;  // per segment: the forward saw no escape or fallback scatter
     void* hpin = nullptr;            // pinned host staging of the differentiable path
     size_t hpin_bytes = 0;
+    // rigid bodies on the grid (set_rigid_spheres / set_prescribed_solids)
+    float4* d_rig = nullptr;     // MPM_MAX_RIGID bodies x 5 rows
+    double* d_wrench = nullptr;  // MPM_MAX_RIGID x 8: last substep's reaction
+    uint32_t rigid_count = 0;
+    bool prescribed = false;
     // soil checkpoint: the archived words the planes do not hold (position and
     // velocity w, C/F column pads, tire compaction; 12 per particle, by id)
     uint32_t* ck_aux = nullptr;
@@ -232,7 +238,7 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap};
+    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap, s->d_rig, s->d_wrench};
     for (void* b : dbufs)
         if (b) hipFree(b);
     if (s->hpin) hipHostFree(s->hpin);
@@ -461,30 +467,44 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = null
     return MPM_OK;
 }
 
-int grid_update(mpm_sim* s, float dt, bool zero_after) {
+// `react`: this substep reports the rigid bodies' reaction (the last
+// substep of a step: the reference resets the wrench every substep, so only
+// the last one is observable).
+int grid_update(mpm_sim* s, float dt, bool zero_after, bool react = false) {
     if (s->dry) return MPM_OK;
     Stage st(s, MPM_STAGE_GRID);
     const int nxt = 1 - s->list_cur;
+    const mpm::GridDev g = griddev(s);
+    uint32_t* cur_count = s->status + mpm::ST_COUNT0 + s->list_cur;
+    const bool rigid = s->rigid_count > 0;
+    const mpm::RigidDev rg{s->d_rig, rigid && react ? s->d_wrench : nullptr, s->rigid_count, s->prescribed ? 1u : 0u};
+    if (rg.wrench) HIP_TRY(s, mpm::launch_zero_f64(s->d_wrench, 8 * s->rigid_count, s->stream));
+    const int parts = s->p2g_mode == MPM_P2G_FUSED ? 1 : 0;
     if (s->jacobi_iters > 0) {
         // Semi-implicit step (mpm.rs:2515-2530): the sweeps need this substep's
         // tile flags, so the grid update keeps them and the clear follows.
-        const mpm::GridDev g = griddev(s);
-        uint32_t* cur_count = s->status + mpm::ST_COUNT0 + s->list_cur;
-        HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count, nullptr, dt, 0,
-                                           s->p2g_mode == MPM_P2G_FUSED ? 1 : 0, grid_blocks(s), s->stream));
+        HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count, nullptr, dt, 0, parts,
+                                           grid_blocks(s), s->stream, nullptr, rigid ? &rg : nullptr));
         HIP_TRY(s, mpm::launch_implicit_sweeps(s->d_scenes, g, s->gv2, s->jacobi_iters, s->list[s->list_cur],
-                                               cur_count, dt, grid_blocks(s), s->stream));
+                                               cur_count, dt, grid_blocks(s), s->stream, rigid ? &rg : nullptr));
+        if (rg.wrench && !s->prescribed)
+            HIP_TRY(s, mpm::launch_rigid_shell(s->d_scenes, g, s->list[s->list_cur], cur_count, dt, rg,
+                                               grid_blocks(s), s->stream));
         if (zero_after) {
             HIP_TRY(s, mpm::launch_clear_tiles(g, s->list[s->list_cur], cur_count, grid_blocks(s), s->stream));
             HIP_TRY(s, mpm::launch_set_u32(s->status + mpm::ST_COUNT0 + nxt, 0u, s->stream));
         }
         return MPM_OK;
     }
-    HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, griddev(s), s->list[s->list_cur],
-                                       s->status + mpm::ST_COUNT0 + s->list_cur,
+    HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count,
                                        zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, dt,
-                                       zero_after ? 1 : 0, s->p2g_mode == MPM_P2G_FUSED ? 1 : 0,
-                                       grid_blocks(s), s->stream));
+                                       zero_after ? 1 : 0, parts, grid_blocks(s), s->stream, nullptr,
+                                       rigid ? &rg : nullptr));
+    // the shell force reads the updated velocities; the current list and its
+    // count stay valid until the next particle pass
+    if (rg.wrench && !s->prescribed)
+        HIP_TRY(s, mpm::launch_rigid_shell(s->d_scenes, g, s->list[s->list_cur], cur_count, dt, rg, grid_blocks(s),
+                                           s->stream));
     return MPM_OK;
 }
 
@@ -513,7 +533,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
             const bool last = k + 1 == substeps;
             if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
             if ((rc = deterministic_p2g(s, dt)) != MPM_OK) return rc;
-            if ((rc = grid_update(s, dt, !last)) != MPM_OK) return rc;
+            if ((rc = grid_update(s, dt, !last, last)) != MPM_OK) return rc;
             if ((rc = particle_pass(s, mpm::MODE_G2P, dt, 1 - s->list_cur)) != MPM_OK) return rc;
             if (!last) s->list_cur = 1 - s->list_cur;
         }
@@ -527,7 +547,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
     if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur)) != MPM_OK) return rc;
     for (uint32_t k = 0; k < substeps; ++k) {
         const bool last = k + 1 == substeps;
-        if ((rc = grid_update(s, dt, !last)) != MPM_OK) return rc;
+        if ((rc = grid_update(s, dt, !last, last)) != MPM_OK) return rc;
         if (fused) {
             if (!last && !s->hold_order && s->sort_every > 0 && s->since_sort >= s->sort_every) {
                 if ((rc = sort_particles(s, true)) != MPM_OK) return rc;
@@ -1315,9 +1335,19 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks_
     return MPM_OK;
 }
 
+// validate_prescribed_step (mpm_prescribed_solids.rs:78-89)
+int check_prescribed_step(mpm_sim* s, float dt) {
+    if (s->rigid_count && s->prescribed &&
+        (s->jacobi_iters > 0 || !(dt > 0.0f) || !std::isfinite(dt) || !std::isfinite(s->cfg[0].gravity_y)))
+        return fail(s, MPM_ERR_INVALID,
+                    "MPM prescribed solids require explicit stepping, finite gravity and positive finite dt");
+    return MPM_OK;
+}
+
 int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps) {
     if (!sim) return MPM_ERR_INVALID;
     if (!std::isfinite(dt) || dt < 0.0f) return fail(sim, MPM_ERR_INVALID, "dt must be finite and >= 0");
+    if (check_prescribed_step(sim, dt) != MPM_OK) return MPM_ERR_INVALID;
     if (sim->n == 0 || substeps == 0) return MPM_OK;
     sim->diff_steps = 0;
     return run_step_n(sim, dt, substeps);
@@ -1326,6 +1356,7 @@ int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps) {
 int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
     if (!sim) return MPM_ERR_INVALID;
     if (!std::isfinite(dt) || dt < 0.0f) return fail(sim, MPM_ERR_INVALID, "dt must be finite and >= 0");
+    if (check_prescribed_step(sim, dt) != MPM_OK) return MPM_ERR_INVALID;
     if (sim->n == 0 || substeps == 0) return MPM_OK;
     sim->diff_steps = 0;
     uint32_t dtb;
@@ -1502,6 +1533,128 @@ int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n) {
     return read_adjoint(sim, field, out);
 }
 
+// ---- rigid bodies on the grid ----------------------------------------------
+
+namespace {
+
+int rigid_handle(mpm_sim* s) {
+    if (s->n_scenes != 1 || s->cfg[0].model != MPM_MODEL_SHADER)
+        return fail(s, MPM_ERR_INVALID, "rigid bodies are defined for single-scene MPM_MODEL_SHADER handles");
+    return MPM_OK;
+}
+
+// Replaces the body table (5 float4 rows per body) and clears the reaction.
+int upload_rigid(mpm_sim* s, const std::vector<float4>& rows, uint32_t n, bool prescribed) {
+    HIP_TRY(s, hipSetDevice(s->device));
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (!s->d_rig) {
+        HIP_TRY(s, dalloc(&s->d_rig, (size_t)mpm::RIGID_ROWS * MPM_MAX_RIGID));
+        HIP_TRY(s, dalloc(&s->d_wrench, (size_t)8 * MPM_MAX_RIGID));
+    }
+    if (n) HIP_TRY(s, hipMemcpy(s->d_rig, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(s, hipMemset(s->d_wrench, 0, (size_t)8 * MPM_MAX_RIGID * sizeof(double)));
+    const bool pres = prescribed && n > 0;
+    if (n != s->rigid_count || pres != s->prescribed) drop_graphs(s);  // launches carry the count and kind
+    s->rigid_count = n;
+    s->prescribed = pres;
+    return MPM_OK;
+}
+
+}  // namespace
+
+int mpm_set_rigid_spheres(mpm_sim* sim, const float* spheres, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int rc = rigid_handle(sim);
+    if (rc != MPM_OK) return rc;
+    if (n > MPM_MAX_RIGID)
+        return fail(sim, MPM_ERR_INVALID, "set_rigid_spheres: n=" + std::to_string(n) + " > max_rigid=" +
+                                              std::to_string(MPM_MAX_RIGID));
+    if (n && !spheres) return fail(sim, MPM_ERR_INVALID, "spheres must not be NULL");
+    std::vector<float4> rows;
+    for (uint32_t r = 0; r < n; ++r) {
+        const float* q = spheres + 7 * (size_t)r;
+        rows.push_back(make_float4(q[0], q[1], q[2], q[3]));
+        rows.push_back(make_float4(q[4], q[5], q[6], 0.f));
+        rows.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+        rows.push_back(make_float4(0.f, 0.f, 0.f, 1.f));
+        rows.push_back(make_float4(0.f, 0.f, 0.f, 0.f));  // kind 0: legacy sphere
+    }
+    return upload_rigid(sim, rows, n, false);
+}
+
+int mpm_set_prescribed_solids(mpm_sim* sim, const mpm_prescribed_solid* solids, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int rc = rigid_handle(sim);
+    if (rc != MPM_OK) return rc;
+    if (n && (sim->jacobi_iters > 0 || !std::isfinite(sim->cfg[0].gravity_y)))
+        return fail(sim, MPM_ERR_INVALID, "MPM prescribed solids require explicit stepping and finite gravity");
+    if (n > MPM_MAX_RIGID) return fail(sim, MPM_ERR_INVALID, "MPM prescribed solid capacity exceeded");
+    if (n && !solids) return fail(sim, MPM_ERR_INVALID, "solids must not be NULL");
+    // MpmPrescribedSolid::rows (mpm_prescribed_solids.rs:34-75)
+    std::vector<float4> rows;
+    for (uint32_t r = 0; r < n; ++r) {
+        const mpm_prescribed_solid& b = solids[r];
+        bool ok = true;
+        for (int k = 0; k < 3; ++k)
+            ok = ok && std::isfinite(b.center[k]) && std::isfinite(b.linear_velocity[k]) &&
+                 std::isfinite(b.angular_velocity[k]);
+        float n2 = 0.f;
+        for (int k = 0; k < 4; ++k) {
+            ok = ok && std::isfinite(b.orientation[k]);
+            n2 += b.orientation[k] * b.orientation[k];
+        }
+        const float norm = std::sqrt(n2);
+        ok = ok && std::isfinite(norm) && std::fabs(norm - 1.0f) <= 1e-4f;
+        float dims[4], radius = 0.f;
+        switch (b.shape) {
+            case MPM_SOLID_SPHERE:
+                dims[0] = dims[1] = dims[2] = b.dims[0];
+                dims[3] = 1.f;
+                radius = b.dims[0];
+                break;
+            case MPM_SOLID_BOX:
+                dims[0] = b.dims[0], dims[1] = b.dims[1], dims[2] = b.dims[2];
+                dims[3] = 2.f;
+                radius = std::sqrt(b.dims[0] * b.dims[0] + b.dims[1] * b.dims[1] + b.dims[2] * b.dims[2]);
+                break;
+            case MPM_SOLID_CYLINDER:
+                dims[0] = b.dims[0], dims[1] = b.dims[1], dims[2] = b.dims[0];
+                dims[3] = 3.f;
+                radius = std::hypot(b.dims[0], b.dims[1]);
+                break;
+            default: ok = false;
+        }
+        if (ok) {
+            ok = std::isfinite(radius);
+            for (int k = 0; k < 3; ++k) ok = ok && std::isfinite(dims[k]) && dims[k] > 0.f;
+        }
+        if (!ok) return fail(sim, MPM_ERR_INVALID, "MPM prescribed solid: invalid geometry or pose");
+        rows.push_back(make_float4(b.center[0], b.center[1], b.center[2], radius));
+        rows.push_back(make_float4(b.linear_velocity[0], b.linear_velocity[1], b.linear_velocity[2], 0.f));
+        rows.push_back(make_float4(b.angular_velocity[0], b.angular_velocity[1], b.angular_velocity[2], 0.f));
+        rows.push_back(make_float4(b.orientation[0] / norm, b.orientation[1] / norm, b.orientation[2] / norm,
+                                   b.orientation[3] / norm));
+        rows.push_back(make_float4(dims[0], dims[1], dims[2], dims[3]));
+    }
+    return upload_rigid(sim, rows, n, true);
+}
+
+int mpm_read_rigid_wrenches(mpm_sim* sim, float* out, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (n != sim->rigid_count) return fail(sim, MPM_ERR_INVALID, "n must equal the rigid body count");
+    if (n == 0) return MPM_OK;
+    if (!out) return fail(sim, MPM_ERR_INVALID, "out must not be NULL");
+    std::vector<double> w((size_t)8 * n);
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    HIP_TRY(sim, hipMemcpy(w.data(), sim->d_wrench, w.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (uint32_t r = 0; r < n; ++r)
+        for (int k = 0; k < 3; ++k) {
+            out[6 * r + k] = float(w[8 * r + k]);
+            out[6 * r + 3 + k] = float(w[8 * r + 4 + k]);
+        }
+    return MPM_OK;
+}
+
 // ---- soil checkpoint (mpm_checkpoint.rs) ----------------------------------
 
 namespace {
@@ -1573,7 +1726,7 @@ int mpm_checkpoint_restore(mpm_sim* sim, const mpm_soil_meta* meta, const uint32
                       k[7] == bits(c.gravity_y) && k[8] == bits(c.youngs) && k[9] == bits(c.poisson) &&
                       k[12] == bits(c.boundary_friction) && k[13] == bits(c.particle_mass) &&
                       k[14] == bits(c.particle_volume) && iters == sim->jacobi_iters;
-    if (!same || meta->count > c.max_particles || meta->count > sim->cap)
+    if (!same || meta->count > c.max_particles || meta->count > sim->cap || sim->rigid_count != 0)
         return fail(sim, MPM_ERR_INVALID,
                     "MPM soil checkpoint: incompatible configuration, capacity or attached rigid bodies");
     // validated: from here on the handle is replaced

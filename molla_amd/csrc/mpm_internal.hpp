@@ -137,6 +137,8 @@ struct StepArgs {
                      // the current positions (the previous pass scattered them)
 };
 
+struct RigidDev;  // rigid bodies on the grid (mpm_rigid.hpp)
+
 // ---- launchers (mpm_kernels.hip) ----
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
                                 hipStream_t s);
@@ -144,7 +146,11 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
                               int zero_after, int use_partials, int blocks, hipStream_t s,
-                              float4* keep = nullptr);
+                              float4* keep = nullptr, const RigidDev* rigid = nullptr);
+// Shell force of legacy rigid spheres on the active tiles (after the update).
+hipError_t launch_rigid_shell(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
+                              const uint32_t* dcount, float dt, const RigidDev& rg, int blocks, hipStream_t s);
+hipError_t launch_zero_f64(double* p, uint32_t n, hipStream_t s);
 hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t* dcount, int blocks,
                               hipStream_t s, float4* also = nullptr);
 // `iters` semi-implicit Jacobi sweeps of g.gv over the active tiles (scratch:
@@ -152,7 +158,7 @@ hipError_t launch_clear_tiles(const GridDev& g, const uint32_t* dlist, uint32_t*
 // mark this substep's dynamic tiles (FLAG_DYN).
 hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, float4* scratch, uint32_t iters,
                                   const uint32_t* dlist, const uint32_t* dcount, float dt, int blocks,
-                                  hipStream_t s);
+                                  hipStream_t s, const RigidDev* rigid = nullptr);
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 hipError_t launch_snap_counters(const uint32_t* status, uint32_t* dst, hipStream_t s);
 hipError_t launch_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s);

@@ -18,6 +18,7 @@
 // f32 atomics instead.
 #include <algorithm>
 
+#include "mpm_rigid.hpp"
 #include "mpm_scatter.hpp"
 
 namespace mpm {
@@ -466,11 +467,13 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
     }
 }
 
+// RIGID: bodies are present (the imprint code stays out of the plain kernel).
+template <bool RIGID>
 __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
-                                                     int zero_after, int use_partials, float4* keep) {
+                                                     int zero_after, int use_partials, float4* keep, RigidDev rg) {
     const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
@@ -483,7 +486,19 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
         float4 q = (flag & FLAG_GMP) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (use_partials) add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
         float4 out;
-        cell_update(sc, q, ix, iy, iz, dt, out);
+        int owner = -1;
+        if (RIGID && rg.count) {
+            // rigid_imprint (mpm.wgsl:937-1006) between the P2G and the update:
+            // a stamped cell keeps the body's velocity (grid_update :1034-1038)
+            float3 force, torque;
+            owner = in_grid(sc, ix, iy, iz) ? rigid_imprint(rg, sc, ix, iy, iz, dt, q, force, torque) : -1;
+            if (rg.wrench) rigid_wrench_add(rg, owner, force, torque);
+        }
+        if (owner >= 0) {
+            out = make_float4(q.y / q.x, q.z / q.x, q.w / q.x, q.x);
+        } else {
+            cell_update(sc, q, ix, iy, iz, dt, out);
+        }
         g.gv[c] = out;
         if (keep) keep[c] = q;  // the P2G total, for the reverse pass
         if (zero_after) {
@@ -499,9 +514,68 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
 
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
-                              int zero_after, int use_partials, int blocks, hipStream_t s, float4* keep) {
-    hipLaunchKernelGGL(k_grid_update, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count, dt,
-                       zero_after, use_partials, keep);
+                              int zero_after, int use_partials, int blocks, hipStream_t s, float4* keep,
+                              const RigidDev* rigid) {
+    const RigidDev rg = rigid ? *rigid : RigidDev{nullptr, nullptr, 0u, 0u};
+    if (rg.count)
+        hipLaunchKernelGGL(k_grid_update<true>, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count,
+                           dt, zero_after, use_partials, keep, rg);
+    else
+        hipLaunchKernelGGL(k_grid_update<false>, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count,
+                           dt, zero_after, use_partials, keep, rg);
+    return hipGetLastError();
+}
+
+// Shell force of legacy rigid spheres (mpm_rigid_force_integrate,
+// mpm.wgsl:1174-1226) after the grid update and the implicit sweeps: every
+// unstamped cell with sand mass within one cell of a sphere's surface (full
+// weight inside, tapering to zero one cell out) pushes on the sphere with
+// -(m w / dt) (v_sphere - v_cell).  One wave per active tile.
+__global__ __launch_bounds__(256) void k_rigid_shell(const SceneDev* __restrict__ scenes, GridDev g,
+                                                     const uint32_t* __restrict__ dlist,
+                                                     const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
+    const uint32_t ns = *g.scount, nd = *dcount;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+        const SceneDev sc = scenes[g.tile_scene[t]];
+        int tx, ty, tz, ix, iy, iz;
+        tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
+        const float4 cv = g.gv[(t << 6) | uint32_t(lane)];
+        const bool live = in_grid(sc, ix, iy, iz) && cv.w > 1.0e-9f && cv.w == cv.w &&
+                          !rigid_masked_legacy(rg, sc, ix, iy, iz);
+        const float3 node = node_position(sc, ix, iy, iz);
+        for (uint32_t r = 0; r < rg.count; ++r) {
+            const float4* b = rg.body + RIGID_ROWS * r;
+            const float3 arm = make_float3(node.x - b[0].x, node.y - b[0].y, node.z - b[0].z);
+            const float d = sqrtf(arm.x * arm.x + arm.y * arm.y + arm.z * arm.z);
+            const bool near = live && b[4].w == 0.0f && !(d > b[0].w + sc.dx);
+            float3 force = make_float3(0.f, 0.f, 0.f), torque = force;
+            if (near) {
+                float w = 1.0f;
+                if (d > b[0].w) w = fmaxf(0.0f, 1.0f - (d - b[0].w) / sc.dx);
+                const float k = -(cv.w * w / fmaxf(dt, 1.0e-6f));
+                force = make_float3(k * (b[1].x - cv.x), k * (b[1].y - cv.y), k * (b[1].z - cv.z));
+                torque = cross3(arm, force);
+            }
+            if (__ballot(near)) rigid_wrench_add(rg, near ? int(r) : -1, force, torque);
+        }
+    }
+}
+
+hipError_t launch_rigid_shell(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
+                              const uint32_t* dcount, float dt, const RigidDev& rg, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_rigid_shell, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, dt, rg);
+    return hipGetLastError();
+}
+
+__global__ void k_zero_f64(double* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+hipError_t launch_zero_f64(double* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 
@@ -519,7 +593,7 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
 __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restrict__ scenes, GridDev g,
                                                         const float4* __restrict__ src, float4* __restrict__ dst,
                                                         const uint32_t* __restrict__ dlist,
-                                                        const uint32_t* __restrict__ dcount, float dt) {
+                                                        const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
     const uint32_t ns = *g.scount, nd = *dcount;
     const int lane = threadIdx.x & 63;
     for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
@@ -530,7 +604,8 @@ __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restri
         const uint32_t c = (t << 6) | uint32_t(lane);
         const float4 me = src[c];
         const float m = me.w;
-        if (m <= 1.0e-9f || m != m) {
+        // empty cells and cells stamped by a legacy sphere keep their velocity
+        if (m <= 1.0e-9f || m != m || rigid_masked_legacy(rg, sc, ix, iy, iz)) {
             dst[c] = me;
             continue;
         }
@@ -584,7 +659,8 @@ __global__ __launch_bounds__(256) void k_copy_tiles(GridDev g, const float4* __r
 
 hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, float4* scratch, uint32_t iters,
                                   const uint32_t* dlist, const uint32_t* dcount, float dt, int blocks,
-                                  hipStream_t s) {
+                                  hipStream_t s, const RigidDev* rigid) {
+    const RigidDev rg = rigid ? *rigid : RigidDev{nullptr, nullptr, 0u, 0u};
     if (iters == 0) return hipSuccess;
     // odd sweep counts start from a copy so the result lands in g.gv
     float4* a = g.gv;
@@ -595,7 +671,7 @@ hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, floa
         b = g.gv;
     }
     for (uint32_t it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_implicit_sweep, dim3(blocks), dim3(256), 0, s, scenes, g, a, b, dlist, dcount, dt);
+        hipLaunchKernelGGL(k_implicit_sweep, dim3(blocks), dim3(256), 0, s, scenes, g, a, b, dlist, dcount, dt, rg);
         float4* t = a;
         a = b;
         b = t;

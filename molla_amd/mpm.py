@@ -128,8 +128,9 @@ def lib():
             raise MpmError(-1, f"{LIB_PATH} is missing: run __graft_entry__.build()")
         L = C.CDLL(LIB_PATH)
         from .checkpoint import SIGS as _CKPT_SIGS
+        from .rigid import SIGS as _RIGID_SIGS
 
-        for name, (res, args) in {**_SIGS, **_CKPT_SIGS}.items():
+        for name, (res, args) in {**_SIGS, **_CKPT_SIGS, **_RIGID_SIGS}.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -139,8 +140,9 @@ def lib():
 
 def exported_symbols() -> List[str]:
     from .checkpoint import SIGS as _CKPT_SIGS
+    from .rigid import SIGS as _RIGID_SIGS
 
-    return list(_SIGS) + list(_CKPT_SIGS)
+    return list(_SIGS) + list(_CKPT_SIGS) + list(_RIGID_SIGS)
 
 
 def scene_config(cfg: MpmConfig, max_particles: Optional[int] = None) -> SceneConfig:
@@ -336,6 +338,37 @@ class MpmSimulation:
 
     def step_n(self, dt: float, substeps: int):
         self._check(lib().mpm_step_n(self._h, float(dt), int(substeps)))
+
+    # -- rigid bodies (mpm.rs:1521-1566, mpm_prescribed_solids.rs:91-140) --
+    def set_rigid_spheres(self, spheres):
+        """``set_rigid_spheres``: [(center xyz, radius, velocity xyz), ...]."""
+        spheres = list(spheres)
+        if len(spheres) > self.cfg.max_rigid:
+            raise MpmError(2, f"set_rigid_spheres: n={len(spheres)} > max_rigid={self.cfg.max_rigid}")
+        a = np.asarray([[*c, r, *v] for c, r, v in spheres], np.float32).reshape(-1, 7)
+        a = np.ascontiguousarray(a)
+        self._check(lib().mpm_set_rigid_spheres(self._h, _ptr(a), len(spheres)))
+        self.rigid_spheres, self.prescribed_solids = spheres, []
+
+    def set_prescribed_solids(self, solids):
+        """``set_prescribed_solids``: replace all prescribed solids (explicit stepping only)."""
+        from .rigid import PrescribedSolid, solid_struct
+
+        solids = list(solids)
+        if len(solids) > self.cfg.max_rigid:
+            raise MpmError(1, "MPM prescribed solid capacity exceeded")
+        arr = (PrescribedSolid * max(len(solids), 1))(*[solid_struct(b) for b in solids])
+        self._check(lib().mpm_set_prescribed_solids(self._h, arr, len(solids)))
+        self.rigid_spheres, self.prescribed_solids = [], solids
+
+    def read_rigid_wrenches(self):
+        """``read_rigid_wrenches_with``: [(force xyz, torque xyz), ...] of the last substep."""
+        n = len(getattr(self, "rigid_spheres", [])) + len(getattr(self, "prescribed_solids", []))
+        out = np.zeros((n, 6), np.float32)
+        self._check(lib().mpm_read_rigid_wrenches(self._h, _ptr(out), n))
+        return [(tuple(map(float, w[:3])), tuple(map(float, w[3:]))) for w in out]
+
+    read_rigid_wrenches_with = read_rigid_wrenches
 
     # -- soil checkpoint (mpm_checkpoint.rs:388-541) --
     def checkpoint_soil(self):

@@ -268,6 +268,44 @@ def step(cfg, st: State, dt: float, substeps: int, want_grid=False):
     return gm, gp
 
 
+def step_rigid(cfg, st: State, dt: float, substeps: int, rows, want_grid=False):
+    """``step`` with rigid bodies; rows: (n_bodies, 20) floats (center + radius,
+    linear velocity, angular velocity, quaternion xyzw, dims + kind).  Returns
+    (wrench (n_bodies, 6) of the last substep, grid_m, grid_p)."""
+    s = scene_from(cfg)
+    G = int(cfg.grid_size)
+    ct = C.c_double if st.dtype == np.float64 else C.c_float
+    rows = np.ascontiguousarray(np.asarray(rows, np.float32).reshape(-1, 20))
+    nb = rows.shape[0]
+    wrench = np.zeros((max(nb, 1), 6))
+    gm = np.zeros(G ** 3, st.dtype) if want_grid else None
+    gp = np.zeros((G ** 3, 3), st.dtype) if want_grid else None
+    fn = lib().orc_step_rigid_f64 if st.dtype == np.float64 else lib().orc_step_rigid_f32
+    rc = fn(C.byref(s), C.c_uint32(st.n), _p(st.x, ct), _p(st.v, ct), _p(st.C, ct), _p(st.F, ct),
+            _p(st.jp, ct), _p(st.mass, ct), _p(st.vol, ct), _p(st.mat, C.c_uint32), _p(st.E, ct),
+            ct(dt), C.c_uint32(substeps), _p(st.status, C.c_uint32), _p(gm, ct), _p(gp, ct),
+            _p(rows, C.c_float), C.c_uint32(nb), _p(wrench, C.c_double))
+    if rc != 0:
+        raise RuntimeError(f"oracle step failed rc={rc}")
+    return wrench[:nb], gm, gp
+
+
+def rigid_grid(cfg, dt: float, rows, grid_m, grid_p):
+    """Grid half of one substep with rigid bodies on a dense P2G grid:
+    returns (m, p after the imprint, v after the update, mask, wrench)."""
+    s = scene_from(cfg)
+    rows = np.ascontiguousarray(np.asarray(rows, np.float32).reshape(-1, 20))
+    nb = rows.shape[0]
+    m = np.ascontiguousarray(np.asarray(grid_m, np.float64).reshape(-1).copy())
+    p = np.ascontiguousarray(np.asarray(grid_p, np.float64).reshape(-1, 3).copy())
+    v = np.zeros_like(p)
+    mask = np.zeros(m.shape, np.uint8)
+    wrench = np.zeros((max(nb, 1), 6))
+    lib().orc_rigid_grid_f64(C.byref(s), C.c_double(dt), _p(rows, C.c_float), C.c_uint32(nb), _p(m, C.c_double),
+                             _p(p, C.c_double), _p(v, C.c_double), _p(mask, C.c_uint8), _p(wrench, C.c_double))
+    return m, p, v, mask, wrench[:nb]
+
+
 def implicit_sweeps(cfg, dt: float, grid_m, grid_v):
     """Jacobi sweeps of the semi-implicit step on a dense grid (m: G^3, v: G^3 x 3)."""
     s = scene_from(cfg)

@@ -483,17 +483,20 @@ template <typename T>
 struct Grid {
     uint32_t G = 0;
     std::vector<T> m, p, v;  // m[G3], p[3 G3], v[3 G3]
+    std::vector<uint8_t> mask;  // g_rigid_mask: cell stamped by a rigid body
     void init(uint32_t g) {
         G = g;
         size_t n = size_t(g) * g * g;
         m.assign(n, T(0));
         p.assign(3 * n, T(0));
         v.assign(3 * n, T(0));
+        mask.assign(n, 0);
     }
     void clear() {
         std::fill(m.begin(), m.end(), T(0));
         std::fill(p.begin(), p.end(), T(0));
         std::fill(v.begin(), v.end(), T(0));
+        std::fill(mask.begin(), mask.end(), uint8_t(0));
     }
     size_t idx(int ix, int iy, int iz) const {
         return size_t(ix) + size_t(G) * (size_t(iy) + size_t(G) * size_t(iz));
@@ -593,6 +596,10 @@ void grid_update_shader(const orc_scene* s, T dt, Grid<T>& g) {
                     continue;
                 }
                 T v[3] = {g.p[3 * c] / m, g.p[3 * c + 1] / m, g.p[3 * c + 2] / m};
+                if (g.mask[c]) {  // rigid-imprinted: no integration (mpm.wgsl:1034-1038)
+                    for (int a = 0; a < 3; ++a) g.v[3 * c + a] = v[a];
+                    continue;
+                }
                 v[1] = v[1] + dt * T(s->gravity_y);
                 const int pad = 2;
                 if (ix < pad) { v[0] = std::max(v[0], T(0)); v[1] *= bf; v[2] *= bf; }
@@ -623,7 +630,7 @@ void implicit_sweeps(const orc_scene* s, T dt, Grid<T>& g) {
                 for (int ix = 0; ix < n; ++ix) {
                     const size_t c = g.idx(ix, iy, iz);
                     const T m = g.m[c];
-                    if (m <= T(1.0e-9) || !(m == m)) {
+                    if (m <= T(1.0e-9) || !(m == m) || g.mask[c]) {
                         for (int a = 0; a < 3; ++a) nv[3 * c + a] = g.v[3 * c + a];
                         continue;
                     }
@@ -757,20 +764,150 @@ bool plasticity_shader(const orc_scene* s, Particles<T>& P, uint32_t p) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// Rigid bodies on the grid.  A body is five rows of four floats: center and
+// bounding radius, linear velocity, angular velocity, unit quaternion (xyzw),
+// shape dims and kind (0 legacy sphere, 1 sphere, 2 box, 3 cylinder along
+// local y).
+struct Rigids {
+    const float* rows = nullptr;
+    uint32_t count = 0;
+    double* wrench = nullptr;  // 6 per body: force, torque (last substep)
+    const float* b(uint32_t r) const { return rows + 20 * size_t(r); }
+};
+
+// prescribed_solid_contains (mpm.wgsl:927-935) and the legacy distance test
+// (:956-960): the arm is rotated into the body frame by the conjugate of q,
+// local = arm + w t + (-q) x t with t = 2 (-q) x arm.
+template <typename T>
+bool rigid_contains(const float* b, const T arm[3]) {
+    const T kind = T(b[19]);
+    auto len3 = [](T x, T y, T z) { return std::sqrt(x * x + y * y + z * z); };
+    if (kind == T(0)) return len3(arm[0], arm[1], arm[2]) < T(b[3]);
+    if (kind == T(1)) return len3(arm[0], arm[1], arm[2]) < T(b[16]);
+    const T iq[3] = {-T(b[12]), -T(b[13]), -T(b[14])}, w = T(b[15]);
+    T t[3] = {T(2) * (iq[1] * arm[2] - iq[2] * arm[1]), T(2) * (iq[2] * arm[0] - iq[0] * arm[2]),
+              T(2) * (iq[0] * arm[1] - iq[1] * arm[0])};
+    const T u[3] = {iq[1] * t[2] - iq[2] * t[1], iq[2] * t[0] - iq[0] * t[2], iq[0] * t[1] - iq[1] * t[0]};
+    T local[3];
+    for (int a = 0; a < 3; ++a) local[a] = arm[a] + w * t[a] + u[a];
+    if (kind == T(2))
+        return std::fabs(local[0]) < T(b[16]) && std::fabs(local[1]) < T(b[17]) && std::fabs(local[2]) < T(b[18]);
+    return std::sqrt(local[0] * local[0] + local[2] * local[2]) < T(b[16]) && std::fabs(local[1]) < T(b[17]);
+}
+
+template <typename T>
+void node_pos(const orc_scene* s, int ix, int iy, int iz, T out[3]) {
+    const int i[3] = {ix, iy, iz};
+    for (int a = 0; a < 3; ++a) out[a] = T(s->origin[a]) + T(i[a] + s->window[a]) * T(s->cell_size);
+}
+
+// rigid_imprint (mpm.wgsl:937-1006) on every cell of the dense grid: the
+// first body containing the node stamps it (legacy spheres with mass 1e6,
+// solids keep the sand mass and skip empty cells); the sand's impulse
+// m_old v - p_old (solids: minus m_old g dt in y) reacts on the body as
+// -impulse / dt (rigid_cell_imprint_wrench, mpm_oracle.rs:54-80).
+template <typename T>
+void rigid_imprint_grid(const orc_scene* s, const Rigids& R, T dt, Grid<T>& g) {
+    const int n = int(g.G);
+    for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ix = 0; ix < n; ++ix) {
+                const size_t c = g.idx(ix, iy, iz);
+                T node[3];
+                node_pos(s, ix, iy, iz, node);
+                for (uint32_t r = 0; r < R.count; ++r) {
+                    const float* b = R.b(r);
+                    const T arm[3] = {node[0] - T(b[0]), node[1] - T(b[1]), node[2] - T(b[2])};
+                    if (!rigid_contains(b, arm)) continue;
+                    const bool solid = b[19] != 0.0f;
+                    T v[3] = {T(b[4]), T(b[5]), T(b[6])};
+                    if (solid) {  // rigid_body_velocity_at_cell, mpm_oracle.rs:82-90
+                        const T om[3] = {T(b[8]), T(b[9]), T(b[10])};
+                        v[0] += om[1] * arm[2] - om[2] * arm[1];
+                        v[1] += om[2] * arm[0] - om[0] * arm[2];
+                        v[2] += om[0] * arm[1] - om[1] * arm[0];
+                    }
+                    const T m_old = g.m[c];
+                    if (solid && m_old <= T(1.0e-9)) break;
+                    const T stamp = solid ? m_old : T(1.0e6);
+                    const T p_old[3] = {g.p[3 * c], g.p[3 * c + 1], g.p[3 * c + 2]};
+                    g.m[c] = stamp;
+                    for (int a = 0; a < 3; ++a) g.p[3 * c + a] = stamp * v[a];
+                    g.mask[c] = 1;
+                    if (m_old > T(1.0e-9) && R.wrench) {
+                        T imp[3];
+                        for (int a = 0; a < 3; ++a) imp[a] = m_old * v[a] - p_old[a];
+                        if (solid) imp[1] -= m_old * T(s->gravity_y) * dt;
+                        const T inv_dt = T(1) / (solid ? dt : std::max(dt, T(1.0e-6)));
+                        const T f[3] = {-imp[0] * inv_dt, -imp[1] * inv_dt, -imp[2] * inv_dt};
+                        double* w = R.wrench + 6 * size_t(r);
+                        for (int a = 0; a < 3; ++a) w[a] += double(f[a]);
+                        w[3] += double(arm[1] * f[2] - arm[2] * f[1]);
+                        w[4] += double(arm[2] * f[0] - arm[0] * f[2]);
+                        w[5] += double(arm[0] * f[1] - arm[1] * f[0]);
+                    }
+                    break;
+                }
+            }
+}
+
+// mpm_rigid_force_integrate (mpm.wgsl:1174-1226), legacy spheres only: every
+// unstamped cell with sand mass within radius + dx of a sphere's center
+// pushes on it with -(m w / dt)(v_sphere - v_cell), w = 1 inside, tapering
+// linearly to 0 one cell outside.
+template <typename T>
+void rigid_shell_grid(const orc_scene* s, const Rigids& R, T dt, const Grid<T>& g) {
+    const int n = int(g.G);
+    const T dx = T(s->cell_size);
+    for (int iz = 0; iz < n; ++iz)
+        for (int iy = 0; iy < n; ++iy)
+            for (int ix = 0; ix < n; ++ix) {
+                const size_t c = g.idx(ix, iy, iz);
+                const T m = g.m[c];
+                if (m <= T(1.0e-9) || !(m == m) || g.mask[c]) continue;
+                T node[3];
+                node_pos(s, ix, iy, iz, node);
+                for (uint32_t r = 0; r < R.count; ++r) {
+                    const float* b = R.b(r);
+                    if (b[19] != 0.0f) continue;
+                    const T arm[3] = {node[0] - T(b[0]), node[1] - T(b[1]), node[2] - T(b[2])};
+                    const T d = std::sqrt(arm[0] * arm[0] + arm[1] * arm[1] + arm[2] * arm[2]);
+                    const T radius = T(b[3]);
+                    if (d > radius + dx) continue;
+                    T w = T(1);
+                    if (d > radius) w = std::max(T(0), T(1) - (d - radius) / dx);
+                    const T k = -(m * w / std::max(dt, T(1.0e-6)));
+                    const T f[3] = {k * (T(b[4]) - g.v[3 * c]), k * (T(b[5]) - g.v[3 * c + 1]),
+                                    k * (T(b[6]) - g.v[3 * c + 2])};
+                    double* wr = R.wrench + 6 * size_t(r);
+                    for (int a = 0; a < 3; ++a) wr[a] += double(f[a]);
+                    wr[3] += double(arm[1] * f[2] - arm[2] * f[1]);
+                    wr[4] += double(arm[2] * f[0] - arm[0] * f[2]);
+                    wr[5] += double(arm[0] * f[1] - arm[1] * f[0]);
+                }
+            }
+}
+
 template <typename T>
 int step_shader(const orc_scene* s, Particles<T>& P, T dt, uint32_t substeps, uint32_t* status,
-                T* out_m, T* out_p) {
+                T* out_m, T* out_p, const Rigids* R = nullptr) {
     Grid<T> g;
     g.init(s->grid_size);
     uint32_t fault = status ? status[0] : 0u;
     uint32_t gathers = status ? status[1] : 0u;
+    const bool rigid = R && R->count > 0;
     for (uint32_t st = 0; st < substeps; ++st) {
         g.clear();
         if (fault == 0) p2g_shader(s, P, dt, g);
+        // the wrench buffer is reset every substep: the last one remains
+        if (rigid && R->wrench) std::fill(R->wrench, R->wrench + 6 * size_t(R->count), 0.0);
+        if (rigid) rigid_imprint_grid(s, *R, dt, g);
         if (out_m && st + 1 == substeps) std::copy(g.m.begin(), g.m.end(), out_m);
         if (out_p && st + 1 == substeps) std::copy(g.p.begin(), g.p.end(), out_p);
         grid_update_shader(s, dt, g);
         implicit_sweeps(s, dt, g);
+        if (rigid && R->wrench) rigid_shell_grid(s, *R, dt, g);
         if (fault == 0) {
             if (P.n > 0) gathers++;
             uint32_t f = 0;
@@ -1111,10 +1248,10 @@ void diff_backward_step(const DiffCtx& c, double dt, const double* x, const doub
 template <typename T>
 int step_dispatch(const orc_scene* s, uint32_t n, T* x, T* v, T* C, T* F, T* jp, const T* mass,
                   const T* vol, const uint32_t* mat, const T* E, T dt, uint32_t substeps,
-                  uint32_t* status, T* gm, T* gp) {
+                  uint32_t* status, T* gm, T* gp, const Rigids* R = nullptr) {
     if (!s || s->grid_size == 0) return -1;
     Particles<T> P{n, x, v, C, F, jp, mass, vol, E, mat};
-    return step_shader(s, P, dt, substeps, status, gm, gp);
+    return step_shader(s, P, dt, substeps, status, gm, gp, R);
 }
 
 }  // namespace
@@ -1275,6 +1412,47 @@ int orc_step_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, f
     if (s && s->model == ORC_MODEL_CPUDIFF) return -2;
     return step_dispatch<float>(s, n, x, v, C, F, jp, mass, vol, mat, E, dt, substeps, status,
                                 grid_m, grid_p);
+}
+
+// orc_step_* with rigid bodies (rows: 20 floats per body, see Rigids).
+int orc_step_rigid_f64(const orc_scene* s, uint32_t n, double* x, double* v, double* C, double* F,
+                       double* jp, const double* mass, const double* vol, const uint32_t* mat,
+                       const double* E, double dt, uint32_t substeps, uint32_t* status, double* grid_m,
+                       double* grid_p, const float* rows, uint32_t n_bodies, double* wrench) {
+    if (!s || s->model != ORC_MODEL_SHADER) return -2;
+    Rigids R{rows, n_bodies, wrench};
+    return step_dispatch<double>(s, n, x, v, C, F, jp, mass, vol, mat, E, dt, substeps, status, grid_m, grid_p,
+                                 &R);
+}
+int orc_step_rigid_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, float* F,
+                       float* jp, const float* mass, const float* vol, const uint32_t* mat,
+                       const float* E, float dt, uint32_t substeps, uint32_t* status, float* grid_m,
+                       float* grid_p, const float* rows, uint32_t n_bodies, double* wrench) {
+    if (!s || s->model != ORC_MODEL_SHADER) return -2;
+    Rigids R{rows, n_bodies, wrench};
+    return step_dispatch<float>(s, n, x, v, C, F, jp, mass, vol, mat, E, dt, substeps, status, grid_m, grid_p,
+                                &R);
+}
+
+// The grid half of one substep with rigid bodies on a caller-supplied dense
+// P2G grid (m[G^3], p[3 G^3], in/out): imprint, grid update, implicit sweeps,
+// shell force.  v: 3 G^3 out; mask: G^3 out; wrench: 6 per body out.
+void orc_rigid_grid_f64(const orc_scene* s, double dt, const float* rows, uint32_t n_bodies, double* m,
+                        double* p, double* v, uint8_t* mask, double* wrench) {
+    Grid<double> g;
+    g.init(s->grid_size);
+    std::copy(m, m + g.m.size(), g.m.begin());
+    std::copy(p, p + g.p.size(), g.p.begin());
+    Rigids R{rows, n_bodies, wrench};
+    std::fill(wrench, wrench + 6 * size_t(n_bodies), 0.0);
+    rigid_imprint_grid(s, R, dt, g);
+    std::copy(g.m.begin(), g.m.end(), m);
+    std::copy(g.p.begin(), g.p.end(), p);
+    grid_update_shader(s, dt, g);
+    implicit_sweeps(s, dt, g);
+    rigid_shell_grid(s, R, dt, g);
+    std::copy(g.v.begin(), g.v.end(), v);
+    std::copy(g.mask.begin(), g.mask.end(), mask);
 }
 
 int orc_p2g_f64(const orc_scene* s, uint32_t n, const double* x, const double* v, const double* C,

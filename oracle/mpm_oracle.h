@@ -77,6 +77,22 @@ void orc_quad_weights_dfx(double fx, double* dw);
 void orc_plastic_project(const orc_scene* s, const orc_material* m, const double* sigma,
                          double jp_in, double* sigma_out, double* jp_out);
 
+/* Rigid bodies on the grid (mpm.wgsl:927-1006, 1174-1226): rows are 20
+ * floats per body -- center + bounding radius, linear velocity, angular
+ * velocity, unit quaternion xyzw, shape dims + kind (0 legacy sphere,
+ * 1 sphere, 2 box, 3 cylinder along local y).  wrench: 6 doubles per body
+ * (force, torque) of the last substep. */
+int orc_step_rigid_f64(const orc_scene* s, uint32_t n, double* x, double* v, double* C, double* F,
+                       double* jp, const double* mass, const double* vol, const uint32_t* mat,
+                       const double* E, double dt, uint32_t substeps, uint32_t* status, double* grid_m,
+                       double* grid_p, const float* rows, uint32_t n_bodies, double* wrench);
+int orc_step_rigid_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, float* F,
+                       float* jp, const float* mass, const float* vol, const uint32_t* mat,
+                       const float* E, float dt, uint32_t substeps, uint32_t* status, float* grid_m,
+                       float* grid_p, const float* rows, uint32_t n_bodies, double* wrench);
+void orc_rigid_grid_f64(const orc_scene* s, double dt, const float* rows, uint32_t n_bodies, double* m,
+                        double* p, double* v, uint8_t* mask, double* wrench);
+
 /* Semi-implicit Jacobi sweeps (scene jacobi_iters) on a dense grid. */
 void orc_implicit_sweeps(const orc_scene* s, double dt, const double* grid_m, double* grid_v);
 
