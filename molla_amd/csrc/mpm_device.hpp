@@ -39,6 +39,11 @@ __device__ __forceinline__ bool finitef(float v) {
     return (__float_as_uint(v) & 0x7f800000u) != 0x7f800000u;
 }
 
+// Accumulates x * 0, which is 0 for every finite x and NaN for an infinite
+// or NaN one: one FMA per value, one test at the end (finite_acc_ok).
+__device__ __forceinline__ float finite_acc(float acc, float x) { return __builtin_fmaf(x, 0.0f, acc); }
+__device__ __forceinline__ bool finite_acc_ok(float acc) { return acc == acc; }
+
 __device__ __forceinline__ double rsqrt_f64(double x) {
     double r = (double)__frsqrt_rn((float)x);
     r = r * (1.5 - 0.5 * x * r * r);
@@ -83,8 +88,13 @@ struct JacobiPair {
     }
 };
 
-// F (row-major f32) -> U, sigma (>= 0), V in f64 with V proper.
-__device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* sg, double* V) {
+// F (row-major f32) -> U, sigma (>= 0), V in f64 with V proper.  The sweeps
+// stop when every off-diagonal entry of the rotated F^T F is below
+// rel_tol * trace: 1e-15 keeps the factors at f64 accuracy (the adjoint uses
+// them in f64); 1e-10 already leaves U orthonormal and sigma exact to far
+// below f32 rounding, and saves a sweep for results that are rounded to f32.
+__device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* sg, double* V,
+                                             double rel_tol = 1.0e-15) {
     // F stays f32 (exact in f64) and is widened where used, so no f64 copy
     // of it is live across the sweeps.
 #define MPM_F(i) ((double)F[i])
@@ -97,7 +107,7 @@ __device__ __forceinline__ void svd3_f64_raw(const float* F, double* U, double* 
     double a12 = MPM_F(1) * MPM_F(2) + MPM_F(4) * MPM_F(5) + MPM_F(7) * MPM_F(8);
     // V stored by columns: v0, v1, v2.
     double v0[3] = {1.0, 0.0, 0.0}, v1[3] = {0.0, 1.0, 0.0}, v2[3] = {0.0, 0.0, 1.0};
-    const double tol = 1.0e-15 * (a00 + a11 + a22);
+    const double tol = rel_tol * (a00 + a11 + a22);
     for (int sweep = 0; sweep < 8; ++sweep) {
         double off = fmax(fabs(a01), fmax(fabs(a02), fabs(a12)));
         if (off <= tol) break;
@@ -163,11 +173,13 @@ __device__ __forceinline__ void svd3_signs(double* U, double* s, double* V) {
     }
 }
 
-// FP64 SVD with the sign convention of MODEL, rounded to f32.
+// FP64 SVD with the sign convention of MODEL, rounded to f32.  The shader
+// model's factors are only used in f32; the differentiable model's forward
+// must match its adjoint's f64 recompute, so it keeps the f64 tolerance.
 template <int MODEL>
 __device__ __forceinline__ void svd3(const float* F, Svd32& o) {
     double U[9], s[3], V[9];
-    svd3_f64_raw(F, U, s, V);
+    svd3_f64_raw(F, U, s, V, MODEL == 0 ? 1.0e-10 : 1.0e-15);
     svd3_signs<MODEL>(U, s, V);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {

@@ -117,12 +117,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
 #pragma unroll
                 for (int k = 0; k < 9; ++k) C[k] = ps[(size_t)(CC + k) * cap + slot];
                 if (MODEL == MODEL_SHADER) {
-                    bool fin = finitef(jp);
+                    float fa = finite_acc(0.0f, jp);
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) fin = fin && finitef(x[k]) && finitef(v[k]);
+                    for (int k = 0; k < 3; ++k) fa = finite_acc(finite_acc(fa, x[k]), v[k]);
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) fin = fin && finitef(C[k]) && finitef(F[k]);
-                    if (!fin) {
+                    for (int k = 0; k < 9; ++k) fa = finite_acc(finite_acc(fa, C[k]), F[k]);
+                    if (!finite_acc_ok(fa)) {
                         latch(a.status, FAULT_NUMERICAL);
                         ok = false;
                     }
@@ -136,15 +136,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 // ---- G2P (mpm.wgsl:1241-1353 / mpm_cpu_diff.rs:544-580) ----
                 float xp[3], fx[3], w[3][3];
                 int b[3];
-                bool fin = finitef(jp);
+                float fa = finite_acc(0.0f, jp);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     xp[k] = (x[k] - sc.origin[k]) * sc.inv_dx;
-                    fin = fin && finitef(x[k]) && finitef(xp[k]);
+                    fa = finite_acc(finite_acc(fa, x[k]), xp[k]);
                 }
 #pragma unroll
-                for (int k = 0; k < 9; ++k) fin = fin && finitef(F[k]);
-                if (MODEL == MODEL_SHADER && !fin) {
+                for (int k = 0; k < 9; ++k) fa = finite_acc(fa, F[k]);
+                if (MODEL == MODEL_SHADER && !finite_acc_ok(fa)) {
                     latch(a.status, FAULT_NUMERICAL);
                     ok = false;
                 }
@@ -231,15 +231,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                         for (int k = 0; k < 9; ++k) Cn[k] *= ad;
                     }
                     float nx[3];
-                    bool fin2 = true;
+                    float fa2 = 0.0f;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
                         nx[k] = x[k] + dt * vn[k];
-                        fin2 = fin2 && finitef(nx[k]) && finitef(vn[k]);
+                        fa2 = finite_acc(finite_acc(fa2, nx[k]), vn[k]);
                     }
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) fin2 = fin2 && finitef(Cn[k]);
-                    if (!fin2) {
+                    for (int k = 0; k < 9; ++k) fa2 = finite_acc(fa2, Cn[k]);
+                    if (!finite_acc_ok(fa2)) {
                         latch(a.status, FAULT_NUMERICAL);
                         ok = false;
                     }
@@ -260,9 +260,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                             for (int j = 0; j < 3; ++j)
                                 Fn[3 * i + j] = F[3 * i + j] + dt * (Cn[3 * i] * F[j] + Cn[3 * i + 1] * F[3 + j] +
                                                                      Cn[3 * i + 2] * F[6 + j]);
+                        float fa3 = 0.0f;
 #pragma unroll
-                        for (int k = 0; k < 9; ++k)
-                            if (!finitef(Fn[k])) ok = false;
+                        for (int k = 0; k < 9; ++k) fa3 = finite_acc(fa3, Fn[k]);
+                        if (!finite_acc_ok(fa3)) ok = false;
                         if (!ok) latch(a.status, FAULT_NUMERICAL);
                     }
                     if (ok) {

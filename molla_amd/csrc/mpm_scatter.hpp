@@ -290,42 +290,51 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             const float ms = p.m * sm;
             const int G = int(sc.G);
             const int li0 = (p.b[0] - x0) + bx * ((p.b[1] - y0) + by * (p.b[2] - z0));
+            // CHECK: test every node against the grid; a wave whose stencils
+            // all lie inside (every particle after the position clamp) skips it
+            auto nodes = [&](auto check) {
+                constexpr bool CHECK = decltype(check)::value;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int iz = p.b[2] + k;
-                if (iz < 0 || iz >= G) continue;
+                for (int k = 0; k < 3; ++k) {
+                    const int iz = p.b[2] + k;
+                    if (CHECK && (iz < 0 || iz >= G)) continue;
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int iy = p.b[1] + j;
-                    if (iy < 0 || iy >= G) continue;
-                    const float wjk = w[1][j] * w[2][k];
-                    const int lrow = li0 + bx * (j + by * k);
-                    float q0 = c[0] + float(j) * a1[0] + float(k) * a2[0];
-                    float q1 = c[1] + float(j) * a1[1] + float(k) * a2[1];
-                    float q2 = c[2] + float(j) * a1[2] + float(k) * a2[2];
+                    for (int j = 0; j < 3; ++j) {
+                        const int iy = p.b[1] + j;
+                        if (CHECK && (iy < 0 || iy >= G)) continue;
+                        const float wjk = w[1][j] * w[2][k];
+                        const int lrow = li0 + bx * (j + by * k);
+                        float q0 = c[0] + float(j) * a1[0] + float(k) * a2[0];
+                        float q1 = c[1] + float(j) * a1[1] + float(k) * a2[1];
+                        float q2 = c[2] + float(j) * a1[2] + float(k) * a2[2];
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const int ix = p.b[0] + i;
-                        if (ix >= 0 && ix < G) {
-                            const float wt = w[0][i] * wjk;
-                            if constexpr (WIDE) {
-                                if (MASS) lds_add_u64(&L.acc[0][lrow + i], fixed64(wt * ms, sm64));
-                                lds_add_u64(&L.acc[1][lrow + i], fixed64(wt * q0, sp64));
-                                lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
-                                lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
-                            } else {
-                                if (MASS) lds_add_i32(&L.acc[0][lrow + i], fixed32(wt, ms));
-                                lds_add_i32(&L.acc[1][lrow + i], fixed32(wt, q0));
-                                lds_add_i32(&L.acc[2][lrow + i], fixed32(wt, q1));
-                                lds_add_i32(&L.acc[3][lrow + i], fixed32(wt, q2));
+                        for (int i = 0; i < 3; ++i) {
+                            const int ix = p.b[0] + i;
+                            if (!CHECK || (ix >= 0 && ix < G)) {
+                                const float wt = w[0][i] * wjk;
+                                if constexpr (WIDE) {
+                                    if (MASS) lds_add_u64(&L.acc[0][lrow + i], fixed64(wt * ms, sm64));
+                                    lds_add_u64(&L.acc[1][lrow + i], fixed64(wt * q0, sp64));
+                                    lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
+                                    lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
+                                } else {
+                                    if (MASS) lds_add_i32(&L.acc[0][lrow + i], fixed32(wt, ms));
+                                    lds_add_i32(&L.acc[1][lrow + i], fixed32(wt, q0));
+                                    lds_add_i32(&L.acc[2][lrow + i], fixed32(wt, q1));
+                                    lds_add_i32(&L.acc[3][lrow + i], fixed32(wt, q2));
+                                }
                             }
+                            q0 += a0[0];
+                            q1 += a0[1];
+                            q2 += a0[2];
                         }
-                        q0 += a0[0];
-                        q1 += a0[1];
-                        q2 += a0[2];
                     }
                 }
-            }
+            };
+            const bool interior = p.b[0] >= 0 && p.b[1] >= 0 && p.b[2] >= 0 && p.b[0] + 2 < G && p.b[1] + 2 < G &&
+                                  p.b[2] + 2 < G;
+            if (__ballot(!interior) == 0) nodes(std::false_type{});
+            else nodes(std::true_type{});
         }
         __syncthreads();
         const double im = 1.0 / sm64, ip = 1.0 / sp64;
