@@ -17,6 +17,11 @@ Prints ONE JSON line.  Besides the headline value it carries:
   cpu_baseline  the FP64 single-core oracle (restatement of the reference CPU
                 path) on a bounded sample of the same workload
   batched       the 512-scene batch (one launch per substep, hipGraph replay)
+  adjoint, batched_adjoint
+                diff_forward with checkpoints, then diff_backward: the
+                adjoint/forward ratio from HIP events around each call's device
+                work with the gradients resident in HBM, and beside it the
+                wall-clock ratio that includes the gradients' host transfers
 """
 from __future__ import annotations
 
@@ -34,6 +39,7 @@ sys.path.insert(0, ROOT)
 METRIC = "MPM particle-substeps/s, plain vs batched, and % HBM roofline; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
 BYTES_FWD = 192.0
+STEP_STAGES = ("sort", "p2g", "grid", "g2p2g", "g2p")
 
 
 def log(msg):
@@ -150,6 +156,32 @@ def run_plain(scale):
             "value": n * 1000 / el, "unit": "particle-substeps/s", "ms_per_step": 1e3 * el / 1000}
 
 
+def timed_diff(sim, dt, steps, every, loss_grad):
+    """One timed diff_forward + diff_backward.  Wall-clock times include the
+    host side of each call (the backward's upload of dJ/dx_T and download of
+    the initial-state gradients); the device times are HIP events on the
+    library's stream around each call's device work, with the gradients
+    resident in HBM (the backward from its uploaded input to its last reverse
+    kernel)."""
+    sim.synchronize()
+    sim.set_profiling(True, stages=False)
+    sim.reset_profile()
+    t0 = time.perf_counter()
+    sim.diff_forward(dt, steps, every)
+    t_fwd = time.perf_counter() - t0
+    J, gx = loss_grad()
+    t0 = time.perf_counter()
+    g = sim.diff_backward(gx)
+    t_bwd = time.perf_counter() - t0
+    prof = sim.read_profile()
+    sim.set_profiling(False)
+    f_dev, b_dev = prof["diff_forward"][0], prof["diff_backward"][0]
+    assert prof["diff_forward"][1] == 1 and prof["diff_backward"][1] == 1
+    return J, g, {"forward_ms": f_dev, "backward_ms": b_dev, "adjoint_fwd_ratio": b_dev / f_dev,
+                  "forward_wall_ms": 1e3 * t_fwd, "backward_wall_ms": 1e3 * t_bwd,
+                  "adjoint_fwd_ratio_wall": t_bwd / t_fwd}
+
+
 def run_adjoint(scale):
     """Forward with checkpoints + reverse pass of the 65,536-particle diff scene
     (64 substeps, checkpoint every 16, COM loss); reports the reverse/forward
@@ -175,14 +207,7 @@ def run_adjoint(scale):
     sim.diff_forward(dt, steps, every)
     sim.diff_backward(loss_grad()[1])
     reset()
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.diff_forward(dt, steps, every)
-    t_fwd = time.perf_counter() - t0
-    J, gx = loss_grad()
-    t0 = time.perf_counter()
-    g = sim.diff_backward(gx)
-    t_bwd = time.perf_counter() - t0
+    J, g, times = timed_diff(sim, dt, steps, every, loss_grad)
     reset()
     sim.synchronize()
     t0 = time.perf_counter()
@@ -191,8 +216,7 @@ def run_adjoint(scale):
     t_plain = time.perf_counter() - t0
     sim.close()
     return {"workload": "diff_scene", "particles": n, "grid": meta["grid"], "substeps": steps,
-            "checkpoint_every": every, "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
-            "adjoint_fwd_ratio": t_bwd / t_fwd, "plain_step_n_ms": 1e3 * t_plain,
+            "checkpoint_every": every, **times, "plain_step_n_ms": 1e3 * t_plain,
             "loss": J, "dJ_dyoungs_sum": float(np.sum(g["E"], dtype=np.float64))}
 
 
@@ -225,22 +249,14 @@ def run_batched_adjoint(scale):
     sim.diff_forward(dt, steps, every)
     sim.diff_backward(loss_grad()[1])
     start(meta["e0"])
-    sim.synchronize()
-    t0 = time.perf_counter()
-    sim.diff_forward(dt, steps, every)
-    t_fwd = time.perf_counter() - t0
-    J, gx = loss_grad()
-    t0 = time.perf_counter()
-    g = sim.diff_backward(gx)
-    t_bwd = time.perf_counter() - t0
+    J, g, times = timed_diff(sim, dt, steps, every, loss_grad)
     sim.close()
     ge = g["youngs_per_scene"]
     # the loss is minimised at E_true: dJ/dE should point from E0 towards it
     agree = float(np.mean(np.sign(ge) == np.sign(np.asarray(meta["e0"]) - np.asarray(meta["e_true"]))))
     return {"workload": "batched_diff_scenes", "scenes": n_scenes, "particles": n, "substeps": steps,
-            "checkpoint_every": every, "grids": [meta["grid_min"], meta["grid_max"]],
-            "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd, "adjoint_fwd_ratio": t_bwd / t_fwd,
-            "forward_value": n * steps / t_fwd, "unit": "particle-substeps/s", "loss": J,
+            "checkpoint_every": every, "grids": [meta["grid_min"], meta["grid_max"]], **times,
+            "forward_value": n * steps / (1e-3 * times["forward_ms"]), "unit": "particle-substeps/s", "loss": J,
             "dJ_dE_sign_agrees_with_E0_minus_Etrue": agree}
 
 
@@ -276,7 +292,7 @@ def main():
         time_steps(sim, dt, args.warmup, args.graph)
     # HIP events around every launch of the timed substeps (graph replays are
     # one opaque launch, so per-kernel events need the eager path)
-    sim.set_profiling(not args.graph)
+    sim.set_profiling(not args.graph, calls=False)
     sim.reset_profile()
     el = time_steps(sim, dt, args.steps, args.graph)
     prof = sim.read_profile()
@@ -293,8 +309,8 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
                 "traffic": traffic, "avg_launch_ms": avg * 1e3, "launches": cnt,
                 "algorithmic_bytes_per_launch": BYTES_FWD * n,
-                "stage_ms_per_substep": {k: (v[0] / args.steps) for k, v in prof.items()},
-                "stage_launches": {k: v[1] for k, v in prof.items()}}
+                "stage_ms_per_substep": {k: (v[0] / args.steps) for k, v in prof.items() if k in STEP_STAGES},
+                "stage_launches": {k: v[1] for k, v in prof.items() if k in STEP_STAGES}}
     log(f"profile: {prof}")
     log(f"status: {st}")
 

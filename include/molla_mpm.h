@@ -195,10 +195,21 @@ typedef enum {
     MPM_STAGE_GRID = 2,
     MPM_STAGE_G2P2G = 3,
     MPM_STAGE_G2P = 4,
-    MPM_STAGE_COUNT = 5
+    /* whole calls: the device work of an mpm_diff_forward, and of an
+     * mpm_diff_backward from the uploaded final-state gradient to the last
+     * reverse kernel (the gradients' host transfers excluded) */
+    MPM_STAGE_DIFF_FORWARD = 5,
+    MPM_STAGE_DIFF_BACKWARD = 6,
+    MPM_STAGE_COUNT = 7
 } mpm_stage;
-int mpm_set_profiling(mpm_sim* sim, int enabled);
-/* Accumulated milliseconds and launch counts per stage since the last reset. */
+/* flags: MPM_PROFILE_STAGES times every launch group of the stages 0-4,
+ * MPM_PROFILE_CALLS the whole-call stages 5-6 (two events per call);
+ * any other nonzero value: both. */
+#define MPM_PROFILE_STAGES 1
+#define MPM_PROFILE_CALLS 2
+int mpm_set_profiling(mpm_sim* sim, int flags);
+/* Accumulated milliseconds and launch counts per stage since the last reset,
+ * MPM_STAGE_COUNT entries each (HIP events on the handle's stream). */
 int mpm_read_profile(mpm_sim* sim, float* ms, uint32_t* launches);
 int mpm_reset_profile(mpm_sim* sim);
 

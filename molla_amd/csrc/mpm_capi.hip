@@ -104,7 +104,7 @@ struct mpm_sim {
     uint32_t pblocks = 0;
     uint32_t gblocks = 0;
     // profiling
-    bool profiling = false;
+    int profiling = 0;         // MPM_PROFILE_* flags
     std::vector<ProfRec> prof_pending;
     double prof_ms[MPM_STAGE_COUNT] = {0};
     uint32_t prof_n[MPM_STAGE_COUNT] = {0};
@@ -122,6 +122,7 @@ struct mpm_sim {
     float* tape = nullptr;     // pre-states of one checkpoint segment (NCOMP planes each)
     float4* gtape = nullptr;   // per substep of the segment: grid v, then P2G total (m, p)
     uint32_t tape_len = 0;
+    int tape_seg = -1;         // the segment the tape holds from the forward itself (-1: none)
     float* adj = nullptr;      // NADJ adjoint planes
     float4* gadj = nullptr;    // per-cell grid adjoint
     double* gml = nullptr;     // per-chunk (dJ/dmu, dJ/dlambda)
@@ -290,19 +291,31 @@ mpm::GridDev griddev(mpm_sim* s) {
 }
 
 // ---- profiling helpers ----
+// Events around a launch group (the stages 0-4) or a whole call (5-6) on
+// the handle's stream.  mark_end() places the end event early (the last
+// call wins); otherwise the destructor places it.
 struct Stage {
     mpm_sim* s;
     ProfRec r{};
-    Stage(mpm_sim* s_, int stage) : s(s_) {
-        if (!s->profiling || s->capturing) return;
+    bool on, ended = false;
+    Stage(mpm_sim* s_, int stage)
+        : s(s_),
+          on(!s_->capturing &&
+             (s_->profiling & (stage >= MPM_STAGE_DIFF_FORWARD ? MPM_PROFILE_CALLS : MPM_PROFILE_STAGES))) {
+        if (!on) return;
         r.stage = stage;
         hipEventCreate(&r.a);
         hipEventCreate(&r.b);
         hipEventRecord(r.a, s->stream);
     }
-    ~Stage() {
-        if (!s->profiling || s->capturing) return;
+    void mark_end() {
+        if (!on) return;
         hipEventRecord(r.b, s->stream);
+        ended = true;
+    }
+    ~Stage() {
+        if (!on) return;
+        if (!ended) hipEventRecord(r.b, s->stream);
         s->prof_pending.push_back(r);
     }
 };
@@ -756,6 +769,66 @@ int diff_check_models(mpm_sim* s) {
     return MPM_OK;
 }
 
+// Room for `every` pre-states and grids.
+int ensure_tape(mpm_sim* s, uint32_t every) {
+    if (s->tape_len >= every) return MPM_OK;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    s->tape_seg = -1;
+    HIP_TRY(s, hipStreamSynchronize(s->stream));
+    if (s->tape) HIP_TRY(s, hipFree(s->tape));
+    if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
+    s->tape = nullptr;
+    s->gtape = nullptr;
+    s->tape_len = 0;
+    HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * s->cap));
+    HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
+    s->tape_len = every;
+    return MPM_OK;
+}
+
+// The last segment of a forward, stepped through the tape: the launches of
+// run_step_n in the fused mode with the order held, except that substep k
+// reads its pre-state from tape slot k and writes the next one to slot k+1
+// (the last G2P writes the live state), and its grid velocity and P2G total
+// go to the grid tape.  The backward then reverses this segment without
+// recomputing it; the live state, the counters and the grid left for
+// read_grid_mass_momentum are those of run_step_n.
+int record_segment(mpm_sim* s, float dt, uint32_t len) {
+    s->w_stepped = true;
+    s->fault_reset = false;
+    int rc = clear_grid(s);
+    if (rc != MPM_OK) return rc;
+    const size_t st_words = (size_t)mpm::NCOMP * s->cap;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    float* live = s->ps[s->ps_cur];
+    HIP_TRY(s, hipMemcpyAsync(s->tape, live, (size_t)24 * s->cap * 4, hipMemcpyDeviceToDevice, s->stream));
+    if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur, s->tape)) != MPM_OK) return rc;
+    const mpm::GridDev g = griddev(s);
+    const int gb = grid_blocks(s);
+    for (uint32_t k = 0; k < len; ++k) {
+        const bool last = k + 1 == len;
+        const int L = s->list_cur;
+        mpm::GridDev gk = g;
+        gk.gv = s->gtape + (size_t)2 * k * cells;
+        HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], s->status + mpm::ST_COUNT0 + L,
+                                           last ? nullptr : s->status + mpm::ST_COUNT0 + (1 - L), dt, last ? 0 : 1,
+                                           1, gb, s->stream, gk.gv + cells));
+        mpm::StepArgs a = args(s, dt, 1 - L);
+        a.ps = s->tape + k * st_words;
+        a.ps_out = last ? live : a.ps + st_words;
+        a.gv = gk.gv;
+        int blocks = 0;
+        use_sorted_chunks(s, a, blocks);
+        HIP_TRY(s, mpm::launch_particle_pass(last ? mpm::MODE_G2P : mpm::MODE_G2P2G, int(s->sh[0].model), a, 1,
+                                             blocks, s->stream));
+        s->since_sort++;
+        if (!last) s->list_cur = 1 - L;
+    }
+    s->boxes_valid = false;
+    s->grid_dirty = true;
+    return MPM_OK;
+}
+
 int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
     if (!std::isfinite(dt) || !(dt > 0.0f)) return fail(s, MPM_ERR_INVALID, "dt must be finite and > 0");
     if (steps == 0 || every == 0) return fail(s, MPM_ERR_INVALID, "steps and checkpoint_every must be >= 1");
@@ -779,29 +852,41 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
         HIP_TRY(s, dalloc(&s->dsnap, snap_words));
         s->dsnap_len = snap_words;
     }
+    if ((rc = ensure_tape(s, every)) != MPM_OK) return rc;
     uint32_t* fsnap = s->dsnap + 8;
     const int mode = s->p2g_mode, wvc = s->write_vc;
     s->p2g_mode = MPM_P2G_FUSED;
     s->write_vc = 1;
     s->diff_steps = 0;
+    s->tape_seg = -1;
     s->adj_valid = false;
+    Stage span(s, MPM_STAGE_DIFF_FORWARD);
+    hipError_t e = hipSuccess;
     rc = clear_grid(s);
     if (rc == MPM_OK) rc = sort_particles(s, true);
     s->hold_order = true;
+    // planes 24.. (jp, mass, volume, E, material, id) do not change in the
+    // CPU-diff model: every tape slot gets them once
+    const size_t st_words = (size_t)mpm::NCOMP * s->cap;
+    for (uint32_t k = 0; rc == MPM_OK && k < every; ++k) {
+        e = hipMemcpyAsync(s->tape + k * st_words + (size_t)24 * s->cap, s->ps[s->ps_cur] + (size_t)24 * s->cap,
+                           (size_t)(mpm::NCOMP - 24) * s->cap * 4, hipMemcpyDeviceToDevice, s->stream);
+        if (e != hipSuccess) rc = hip_fail(s, e, "tape copy");
+    }
     // The escape and fallback counters at every segment boundary tell the
     // backward which segments it may reverse with the fused kernel, so it
     // needs no host round trip per segment (the recompute repeats them).
-    hipError_t e = hipSuccess;
     if (rc == MPM_OK && (e = mpm::launch_snap_counters(s->status, fsnap, s->stream)) != hipSuccess)
         rc = hip_fail(s, e, "counter snapshot");
     for (uint32_t seg = 0; rc == MPM_OK && seg < nseg; ++seg) {
         const uint32_t len = std::min(every, steps - seg * every);
         e = hipMemcpyAsync(s->ckpt[seg], s->ps[s->ps_cur], ck * 4, hipMemcpyDeviceToDevice, s->stream);
         if (e != hipSuccess) rc = hip_fail(s, e, "checkpoint copy");
-        if (rc == MPM_OK) rc = run_step_n(s, dt, len);
+        if (rc == MPM_OK) rc = seg + 1 < nseg ? run_step_n(s, dt, len) : record_segment(s, dt, len);
         if (rc == MPM_OK && (e = mpm::launch_snap_counters(s->status, fsnap + 2 * (seg + 1), s->stream)) != hipSuccess)
             rc = hip_fail(s, e, "counter snapshot");
     }
+    span.mark_end();
     s->hold_order = false;
     s->p2g_mode = mode;
     s->write_vc = wvc;
@@ -813,6 +898,7 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
     for (uint32_t seg = 0; seg < nseg; ++seg)
         s->seg_fused[seg] = snap[2 * seg] == snap[2 * seg + 2] && snap[2 * seg + 1] == snap[2 * seg + 3];
     s->diff_steps = steps;
+    s->tape_seg = int(nseg) - 1;
     s->diff_every = every;
     s->diff_dt = dt;
     s->diff_sorts = s->sorts;
@@ -846,10 +932,66 @@ int ensure_pinned(mpm_sim* s, size_t bytes) {
 
 constexpr int BACKWARD_REPLAN = -1;  // a segment's recompute escaped where its forward did not
 
+// Recomputes segment `seg` from its checkpoint with the fused forward:
+// substep k's pre-state goes to tape slot k and its grid velocity and P2G
+// total to the grid tape.  L: the list the segment's first P2G fills; on
+// return, the list its last grid update consumed.  `fused`: whether no
+// particle escaped to the global atomics (read back when `checked`, else the
+// forward's plan, with the counters snapshotted for the check at the end).
+int recompute_segment(mpm_sim* s, uint32_t seg, uint32_t len, bool checked, uint32_t* bsnap, int& L, bool& fused) {
+    const float dt = s->diff_dt;
+    const size_t st_words = (size_t)mpm::NCOMP * s->cap;
+    const size_t cells = (size_t)s->total_tiles * 64;
+    const mpm::GridDev g = griddev(s);
+    const int gb = grid_blocks(s);
+    auto count = [&](int l) { return s->status + mpm::ST_COUNT0 + l; };
+    uint32_t esc0 = 0, fb0 = 0;
+    if (checked) {
+        uint32_t w0[mpm::ST_WORDS];
+        HIP_TRY(s, hipMemcpyAsync(w0, s->status, sizeof(w0), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        esc0 = w0[mpm::ST_ESCAPES];
+        fb0 = w0[mpm::ST_FALLBACK];
+    } else {
+        HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg, s->stream));
+    }
+    float* t0 = s->tape;
+    HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], (size_t)24 * s->cap * 4, hipMemcpyDeviceToDevice, s->stream));
+    int rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0);
+    if (rc != MPM_OK) return rc;
+    for (uint32_t k = 0; k < len; ++k) {
+        float* tk = s->tape + k * st_words;
+        mpm::GridDev gk = g;
+        gk.gv = s->gtape + (size_t)2 * k * cells;
+        HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
+                                           s->stream, gk.gv + cells));
+        if (k + 1 == len) break;
+        mpm::StepArgs a = args(s, dt, 1 - L);
+        a.ps = tk;
+        a.ps_out = tk + st_words;  // state k+1 goes straight to the next tape slot
+        a.gv = gk.gv;
+        a.write_vc = 1;
+        int blocks = 0;
+        use_sorted_chunks(s, a, blocks);
+        HIP_TRY(s, mpm::launch_particle_pass(mpm::MODE_G2P2G, int(s->sh[0].model), a, 1, blocks, s->stream));
+        L = 1 - L;
+    }
+    if (checked) {
+        uint32_t w1[mpm::ST_WORDS];
+        HIP_TRY(s, hipMemcpyAsync(w1, s->status, sizeof(w1), hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(s, hipStreamSynchronize(s->stream));
+        fused = w1[mpm::ST_ESCAPES] == esc0 && w1[mpm::ST_FALLBACK] == fb0;
+    } else {
+        HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg + 2, s->stream));
+        fused = s->seg_fused[seg] != 0;
+    }
+    return MPM_OK;
+}
+
 // One reverse sweep.  `checked`: read the escape counters around every
 // segment's recompute on the host (two synchronisations per segment) instead
 // of using the plan the forward recorded.
-int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checked) {
+int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
     const uint32_t cap = s->cap, every = s->diff_every, steps = s->diff_steps;
     const size_t plane = cap;
     const size_t cells = (size_t)s->total_tiles * 64;
@@ -866,12 +1008,8 @@ int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checke
     HIP_TRY(s, mpm::launch_copy_u32(s->status + mpm::ST_GATHERS, s->dsnap, s->stream));
     HIP_TRY(s, hipMemsetAsync(s->adj, 0, (size_t)mpm::NADJ * plane * 4, s->stream));
     HIP_TRY(s, hipMemsetAsync(s->gml, 0, 2 * (size_t)s->max_schunks * sizeof(double), s->stream));
-    const size_t vec = (size_t)s->n * 12;
-    std::memcpy(s->hpin, gx, vec);
-    if (gv) std::memcpy(static_cast<char*>(s->hpin) + vec, gv, vec);
-    HIP_TRY(s, hipMemcpyAsync(s->staging, s->hpin, gv ? 2 * vec : vec, hipMemcpyHostToDevice, s->stream));
     HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AX, 3, s->slot_of_id, s->staging, s->n, s->stream));
-    if (gv)
+    if (has_gv)
         HIP_TRY(s, mpm::launch_permute_in(s->adj, cap, mpm::AV, 3, s->slot_of_id, s->staging + 3 * (size_t)s->n,
                                           s->n, s->stream));
     const float dt = s->diff_dt;
@@ -880,63 +1018,27 @@ int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checke
     const int gb = grid_blocks(s);
     int L = s->list_cur;
     auto count = [&](int l) { return s->status + mpm::ST_COUNT0 + l; };
-    // planes 24.. (jp, mass, volume, E, material, id) do not change in the
-    // CPU-diff model: every tape slot gets them once
-    for (uint32_t k = 0; k < every; ++k)
-        HIP_TRY(s, hipMemcpyAsync(s->tape + k * st_words + 24 * plane, s->ps[s->ps_cur] + 24 * plane,
-                                  (size_t)(mpm::NCOMP - 24) * plane * 4, hipMemcpyDeviceToDevice, s->stream));
+    // the tape's planes 24.. hold the forward's constant planes
+    int kept = -1;
     for (int seg = int(nseg) - 1; seg >= 0 && rc == MPM_OK; --seg) {
         const uint32_t len = std::min(every, steps - uint32_t(seg) * every);
-        uint32_t esc0 = 0, fb0 = 0;
-        if (checked) {
-            uint32_t w0[mpm::ST_WORDS];
-            HIP_TRY(s, hipMemcpyAsync(w0, s->status, sizeof(w0), hipMemcpyDeviceToHost, s->stream));
-            HIP_TRY(s, hipStreamSynchronize(s->stream));
-            esc0 = w0[mpm::ST_ESCAPES];
-            fb0 = w0[mpm::ST_FALLBACK];
-        } else {
-            HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg, s->stream));
-        }
-        float* t0 = s->tape;
-        HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], 24 * plane * 4, hipMemcpyDeviceToDevice, s->stream));
-        // Recompute the segment with the fused forward: substep k's grid goes
-        // to the grid tape and its pre-state to tape slot k.
-        if ((rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0)) != MPM_OK) break;
-        for (uint32_t k = 0; k < len && rc == MPM_OK; ++k) {
-            float* tk = s->tape + k * st_words;
-            mpm::GridDev gk = g;
-            gk.gv = s->gtape + (size_t)2 * k * cells;
-            HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
-                                               s->stream, gk.gv + cells));
-            if (k + 1 == len) break;
-            float* tk1 = tk + st_words;
-            mpm::StepArgs a = args(s, dt, 1 - L);
-            a.ps = tk;
-            a.ps_out = tk1;  // state k+1 goes straight to the next tape slot
-            a.gv = gk.gv;
-            a.write_vc = 1;
-            int blocks = 0;
-            use_sorted_chunks(s, a, blocks);
-            HIP_TRY(s, mpm::launch_particle_pass(mpm::MODE_G2P2G, int(s->sh[0].model), a, 1, blocks, s->stream));
-            L = 1 - L;
-        }
-        if (rc != MPM_OK) break;
-        // The last grid update reset the other list; the reverse builds its
-        // own list of escaped tiles there.
-        const int R = 1 - L;
         // No particle of the segment scattered with global atomics (its reverse
         // scatters use the same stencils): the reverse P2G of substep k and the
         // reverse G2P of substep k-1 run as one pass.
         bool fused;
-        if (checked) {
-            uint32_t w1[mpm::ST_WORDS];
-            HIP_TRY(s, hipMemcpyAsync(w1, s->status, sizeof(w1), hipMemcpyDeviceToHost, s->stream));
-            HIP_TRY(s, hipStreamSynchronize(s->stream));
-            fused = w1[mpm::ST_ESCAPES] == esc0 && w1[mpm::ST_FALLBACK] == fb0;
-        } else {
-            HIP_TRY(s, mpm::launch_snap_counters(s->status, bsnap + 4 * seg + 2, s->stream));
+        if (seg == s->tape_seg) {
+            // the forward stepped this segment through the tape: its last grid
+            // update consumed list L and left the other list's count behind
+            kept = seg;
+            HIP_TRY(s, mpm::launch_set_u32(count(1 - L), 0u, s->stream));
             fused = s->seg_fused[seg] != 0;
+        } else {
+            s->tape_seg = -1;  // the recompute overwrites the tape
+            if ((rc = recompute_segment(s, uint32_t(seg), len, checked, bsnap, L, fused)) != MPM_OK) break;
         }
+        // The last grid update reset the other list; the reverse builds its
+        // own list of escaped tiles there.
+        const int R = 1 - L;
         auto tape_at = [&](int k) { return s->tape + (size_t)k * st_words; };
         auto gv_at = [&](int k) { return s->gtape + (size_t)2 * k * cells; };
         if (fused) {
@@ -978,6 +1080,7 @@ int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checke
         }
         L = R;
     }
+    span.mark_end();
     s->list_cur = L;
     s->grid_dirty = false;
     s->p2g_mode = mode;
@@ -990,7 +1093,8 @@ int diff_backward_pass(mpm_sim* s, const float* gx, const float* gv, bool checke
     if (!checked) {
         const uint32_t* w = static_cast<const uint32_t*>(s->hpin);
         for (uint32_t seg = 0; seg < nseg; ++seg)
-            if (s->seg_fused[seg] && (w[4 * seg] != w[4 * seg + 2] || w[4 * seg + 1] != w[4 * seg + 3]))
+            if (int(seg) != kept && s->seg_fused[seg] &&
+                (w[4 * seg] != w[4 * seg + 2] || w[4 * seg + 1] != w[4 * seg + 3]))
                 return BACKWARD_REPLAN;
     }
     return MPM_OK;
@@ -1004,17 +1108,6 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     const uint32_t every = s->diff_every, steps = s->diff_steps;
     const uint32_t nseg = (steps + every - 1) / every;
     const size_t cells = (size_t)s->total_tiles * 64;
-    if (s->tape_len < every) {
-        HIP_TRY(s, hipStreamSynchronize(s->stream));
-        if (s->tape) HIP_TRY(s, hipFree(s->tape));
-        if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
-        s->tape = nullptr;
-        s->gtape = nullptr;
-        s->tape_len = 0;
-        HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * s->cap));
-        HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
-        s->tape_len = every;
-    }
     if (!s->adj) HIP_TRY(s, dalloc(&s->adj, (size_t)mpm::NADJ * s->cap));
     if (!s->gml) HIP_TRY(s, dalloc(&s->gml, 2 * (size_t)s->max_schunks));
     if (!s->gadj) {
@@ -1027,8 +1120,14 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     int rc = ensure_pinned(s, pin);
     if (rc != MPM_OK) return rc;
     s->adj_valid = false;
-    rc = diff_backward_pass(s, gx, gv, false);
-    if (rc == BACKWARD_REPLAN) rc = diff_backward_pass(s, gx, gv, true);
+    // the final-state gradients to the staging buffer, in the caller's order
+    const size_t vec = (size_t)s->n * 12;
+    std::memcpy(s->hpin, gx, vec);
+    if (gv) std::memcpy(static_cast<char*>(s->hpin) + vec, gv, vec);
+    HIP_TRY(s, hipMemcpyAsync(s->staging, s->hpin, gv ? 2 * vec : vec, hipMemcpyHostToDevice, s->stream));
+    Stage span(s, MPM_STAGE_DIFF_BACKWARD);
+    rc = diff_backward_pass(s, gv != nullptr, false, span);
+    if (rc == BACKWARD_REPLAN) rc = diff_backward_pass(s, gv != nullptr, true, span);
     if (rc != MPM_OK) return rc;
     s->adj_valid = true;
     return MPM_OK;
@@ -1107,8 +1206,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     HIP_TRY(s, hipMemcpyAsync(st, s->status, sizeof(st), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     const uint32_t sorts0 = s->sorts;
-    const bool prof = s->profiling;
-    s->profiling = false;
+    const int prof = s->profiling;
+    s->profiling = 0;
     hipEvent_t ea = nullptr, eb = nullptr;
     auto restore = [&]() -> int {
         int r = clear_grid(s);
@@ -1482,7 +1581,8 @@ int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_
 
 int mpm_set_profiling(mpm_sim* sim, int enabled) {
     if (!sim) return MPM_ERR_INVALID;
-    sim->profiling = enabled != 0;
+    const int all = MPM_PROFILE_STAGES | MPM_PROFILE_CALLS;
+    sim->profiling = (enabled & ~all) ? all : enabled;
     return MPM_OK;
 }
 

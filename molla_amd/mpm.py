@@ -37,7 +37,8 @@ _P2G_NAMES = {"fused": P2G_FUSED, "two_pass": P2G_FUSED, "global_atomic": P2G_GL
 FIELDS = {"position": (0, 3), "velocity": (1, 3), "C": (2, 9), "F": (3, 9), "jp": (4, 1),
           "mass": (5, 1), "volume": (6, 1), "youngs": (7, 1), "material": (8, 1)}
 
-STAGES = ["sort", "p2g", "grid", "g2p2g", "g2p"]
+STAGES = ["sort", "p2g", "grid", "g2p2g", "g2p", "diff_forward", "diff_backward"]
+PROFILE_STAGES, PROFILE_CALLS = 1, 2
 
 
 class MpmError(RuntimeError):
@@ -423,8 +424,11 @@ class MpmSimulation:
         return keys, ids
 
     # -- profiling --
-    def set_profiling(self, enabled: bool):
-        self._check(lib().mpm_set_profiling(self._h, 1 if enabled else 0))
+    def set_profiling(self, enabled: bool, stages: bool = True, calls: bool = True):
+        """HIP events per launch group of the step stages (`stages`) and around
+        whole diff_forward / diff_backward calls (`calls`)."""
+        flags = (PROFILE_STAGES if stages else 0) | (PROFILE_CALLS if calls else 0)
+        self._check(lib().mpm_set_profiling(self._h, flags if enabled else 0))
 
     def read_profile(self) -> dict:
         ms = np.zeros(len(STAGES), np.float32)

@@ -221,3 +221,72 @@ def test_adjoint_with_escaped_particles_matches_oracle():
         e = rel_err(g[k], o[k])
         log(f"  d/d{k}0 err {e:.2e}")
         assert e < 1e-4, k
+
+
+@pytest.mark.parametrize("steps,every", [(20, 8), (12, 12)])
+def test_forward_taped_segment_matches_step_n_and_repeats(steps, every):
+    # diff_forward steps its last segment through the tape (no recompute in
+    # the first backward after it): the final state, the grid left for
+    # read_grid_mass_momentum and the status equal those of step_n over the
+    # same slot order (one sort at the start: the float partials and escape
+    # atomics make the rounding depend on the order), and a second backward
+    # (which recomputes every segment when there are several) returns the
+    # same gradients bit for bit.
+    cfg, d, meta = _scene(n=2048, seed=41)
+    d = _perturbed(d, 42)
+    n, dt = len(d["x"]), meta["dt"]
+    sims = []
+    for diff in (True, False):
+        sim = MpmSimulation(cfg, max_particles=n)
+        sim.set_particles(d["x"], d["v"])
+        sim.write_field("youngs", d["E"])
+        sim.write_field("F", d["F"].reshape(-1, 9))
+        if diff:
+            sim.diff_forward(dt, steps, every)
+        else:
+            sim.set_sort_interval(0)
+            sim.step_n(dt, steps)
+        sims.append(sim)
+    a, b = sims
+    log(f"taped forward: n={n} steps={steps} every={every}")
+    for name in ("position", "velocity", "C", "F"):
+        assert np.array_equal(a.read_field(name), b.read_field(name)), name
+    ma, pa = a.read_grid_mass_momentum()
+    mb, pb = b.read_grid_mass_momentum()
+    assert np.array_equal(ma, mb) and np.array_equal(pa, pb)
+    assert ma.sum() > 0
+    sa, sb = a.read_status(), b.read_status()
+    for k in ("fault_bits", "completed_gathers", "escapes", "sorts"):
+        assert sa[k] == sb[k], k
+    rng = np.random.default_rng(43)
+    gx = rng.standard_normal((n, 3)).astype(np.float32)
+    g1 = a.diff_backward(gx)
+    g2 = a.diff_backward(gx)
+    for k in ("x", "v", "E"):
+        assert np.array_equal(g1[k], g2[k]), k
+    assert g1["mu"] == g2["mu"] and g1["lambda"] == g2["lambda"]
+    for s in sims:
+        s.close()
+
+
+def test_profiling_spans_of_the_diff_calls():
+    cfg, d, meta = _scene(n=2048, seed=44)
+    n, dt = len(d["x"]), meta["dt"]
+    sim = MpmSimulation(cfg, max_particles=n)
+    sim.set_particles(d["x"], d["v"])
+    sim.set_profiling(True, stages=False)
+    sim.step_n(dt, 4)
+    sim.diff_forward(dt, 16, 8)
+    sim.diff_backward(np.ones((n, 3), np.float32))
+    p = sim.read_profile()
+    log(f"calls only: {p}")
+    assert all(p[s][1] == 0 for s in ("sort", "p2g", "grid", "g2p2g", "g2p"))
+    assert p["diff_forward"][1] == 1 and p["diff_backward"][1] == 1
+    assert p["diff_forward"][0] > 0 and p["diff_backward"][0] > 0
+    sim.reset_profile()
+    sim.set_profiling(True)
+    sim.step_n(dt, 4)
+    p = sim.read_profile()
+    assert p["grid"][1] == 4 and p["diff_forward"][1] == 0
+    sim.set_profiling(False)
+    sim.close()
