@@ -15,7 +15,11 @@ Prints ONE JSON line.  Besides the headline value it carries:
                 taken from the FETCH_SIZE/WRITE_SIZE counter summary in
                 profiles/, scaled by the same counters on a copy of known size
   cpu_baseline  the FP64 single-core oracle (restatement of the reference CPU
-                path) on a bounded sample of the same workload
+                path) on a bounded sample of the same workload, with the
+                GPU's relative error against it on that sample
+  cpu_baseline_fair
+                the f32 threaded (OpenMP) port of the oracle on every host
+                thread, same sample
   batched       the 512-scene batch (one launch per substep, hipGraph replay)
   adjoint, batched_adjoint
                 diff_forward with checkpoints, then diff_backward: the
@@ -109,15 +113,34 @@ def time_steps(sim, dt, k, graph):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(cfg, d, dt, substeps):
+def cpu_baseline(cfg, d, dt, substeps, gpu_state):
+    """The CPU legs on one bounded sample (the first `substeps` substeps of the
+    workload): the FP64 oracle on one core (faithful), the f32 threaded port
+    on every host thread (fair), and, with the FP64 result as the checker,
+    the relative error of the GPU's state after the same substeps."""
     import oracle as orc
 
     n = len(d["x"])
-    st = orc.State(d["x"], d["v"], mass=d["mass"], vol=d["volume"], mat=d["material"], dtype=np.float64)
+
+    def state(dtype):
+        return orc.State(d["x"], d["v"], mass=d["mass"], vol=d["volume"], mat=d["material"], dtype=dtype)
+
+    st = state(np.float64)
     t0 = time.perf_counter()
     orc.step(cfg, st, dt, substeps)
     el = time.perf_counter() - t0
-    return n * substeps / el, el
+    s32 = state(np.float32)
+    t0 = time.perf_counter()
+    threads = orc.step_threads(cfg, s32, dt, substeps)
+    el_fair = time.perf_counter() - t0
+
+    def rel(a, b):
+        return float(np.abs(a.astype(np.float64) - b).max() / max(np.abs(b).max(), 1e-30))
+
+    err = {f: rel(gpu_state[f], getattr(st, f)) for f in ("x", "v", "F")}
+    err32 = {f: rel(getattr(s32, f), getattr(st, f)) for f in ("x", "v", "F")}
+    return {"value": n * substeps / el, "seconds": el, "fair_value": n * substeps / el_fair,
+            "fair_seconds": el_fair, "threads": threads, "gpu_error": err, "fair_error": err32}
 
 
 def run_batched(steps, warmup, scale, autotune=True):
@@ -355,11 +378,24 @@ def main():
     if not args.no_cpu:
         sim.close()
         k = args.cpu_substeps
-        rate, el_cpu = cpu_baseline(cfg, d, dt, k)
-        out["cpu_baseline"] = {"value": rate, "unit": "particle-substeps/s", "cores": 1,
+        # the GPU's state after the sample's substeps, from the same start
+        from molla_amd import MpmSimulation
+        gsim, _, _, _ = build_large(MpmSimulation, args.scale)
+        gsim.step_n(dt, k)
+        gpu_state = {"x": gsim.read_positions(), "v": gsim.read_velocities(),
+                     "F": gsim.read_field("F").reshape(-1, 3, 3)}
+        gsim.close()
+        cb = cpu_baseline(cfg, d, dt, k, gpu_state)
+        sample = f"first {k} substeps of large_scene ({n} particles)"
+        out["cpu_baseline"] = {"value": cb["value"], "unit": "particle-substeps/s", "cores": 1,
                                "kind": "port",
-                               "sample": f"first {k} substeps of large_scene ({n} particles), FP64 oracle, "
-                                         f"{el_cpu:.1f} s"}
+                               "sample": f"{sample}, FP64 oracle, {cb['seconds']:.1f} s",
+                               "gpu_rel_error_vs_fp64": cb["gpu_error"]}
+        out["cpu_baseline_fair"] = {"value": cb["fair_value"], "unit": "particle-substeps/s",
+                                    "cores": cb["threads"], "kind": "port",
+                                    "sample": f"{sample}, f32 threaded port of the oracle, "
+                                              f"{cb['fair_seconds']:.2f} s",
+                                    "rel_error_vs_fp64": cb["fair_error"]}
     print(json.dumps(out), flush=True)
 
 

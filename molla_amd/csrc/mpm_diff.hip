@@ -437,8 +437,7 @@ __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ s
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
         if (m > 0.0f) {
             float gx = q.y, gy = q.z, gz = q.w;
-            const float node_y = sc.origin[1] + float(iy) * sc.dx;
-            if (node_y < sc.floor_y) {
+            if (below_floor(sc, iy)) {
                 float vy = mp.z / m;  // same operations as the forward's floor test
                 vy += sc.gravity_y * dt;
                 if (vy < 0.f) gy = 0.f;

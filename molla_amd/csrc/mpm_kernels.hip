@@ -456,8 +456,7 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
         if (m > 0.0f) {
             float vx = q.y / m, vy = q.z / m, vz = q.w / m;
             vy += sc.gravity_y * dt;
-            const float node_y = sc.origin[1] + float(iy) * sc.dx;
-            if (node_y < sc.floor_y) {
+            if (below_floor(sc, iy)) {
                 if (vy < 0.f) vy = 0.f;
                 const float damp = 1.0f - sc.floor_friction;
                 vx *= damp;

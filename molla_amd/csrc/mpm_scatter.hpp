@@ -35,6 +35,14 @@ __device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int i
     return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
 }
 
+// The CPU-diff floor (mpm_cpu_diff.rs:527-531): node row iy lies strictly
+// below floor_y.  Evaluated in f64 from the scene's f32 values, where the
+// product is exact, so a row whose f32 position would round onto floor_y is
+// decided as the f64 oracle decides it (e.g. dx = 1/80, floor_y = 3 dx).
+__device__ __forceinline__ bool below_floor(const SceneDev& sc, int iy) {
+    return (double)sc.origin[1] + (double)iy * (double)sc.dx < (double)sc.floor_y;
+}
+
 __device__ __forceinline__ bool in_grid(const SceneDev& sc, int ix, int iy, int iz) {
     int n = int(sc.G);
     return ix >= 0 && iy >= 0 && iz >= 0 && ix < n && iy < n && iz < n;

@@ -268,6 +268,24 @@ def step(cfg, st: State, dt: float, substeps: int, want_grid=False):
     return gm, gp
 
 
+def step_threads(cfg, st: State, dt: float, substeps: int) -> int:
+    """The SHADER step in f32 on all host threads (OpenMP), in place: a timing
+    port for the CPU baseline, whose sums run in another order than ``step``'s.
+    Returns the thread count used."""
+    if st.dtype != np.float32:
+        raise ValueError("step_threads runs in f32")
+    s = scene_from(cfg)
+    ct = C.c_float
+    threads = C.c_uint32(0)
+    rc = lib().orc_step_threads_f32(C.byref(s), C.c_uint32(st.n), _p(st.x, ct), _p(st.v, ct), _p(st.C, ct),
+                                    _p(st.F, ct), _p(st.jp, ct), _p(st.mass, ct), _p(st.vol, ct),
+                                    _p(st.mat, C.c_uint32), _p(st.E, ct), ct(dt), C.c_uint32(substeps),
+                                    _p(st.status, C.c_uint32), C.byref(threads))
+    if rc != 0:
+        raise RuntimeError(f"oracle threaded step failed rc={rc}")
+    return int(threads.value)
+
+
 def step_rigid(cfg, st: State, dt: float, substeps: int, rows, want_grid=False):
     """``step`` with rigid bodies; rows: (n_bodies, 20) floats (center + radius,
     linear velocity, angular velocity, quaternion xyzw, dims + kind).  Returns

@@ -34,6 +34,8 @@
 #include <numeric>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 template <typename T>
@@ -542,13 +544,14 @@ M3<T> particle_stress_shader(const orc_scene* s, const Particles<T>& P, uint32_t
     return shader_stress(load9(P.F + 9 * size_t(p)), mu_h, lam_h, dt, P.vol[p]);
 }
 
-// P2G in the SHADER model: mpm.wgsl:494-628 (math shared with p2g_two_pass).
+// P2G of particle p in the SHADER model: mpm.wgsl:494-628 (math shared
+// with p2g_two_pass).
 template <typename T>
-void p2g_shader(const orc_scene* s, const Particles<T>& P, T dt, Grid<T>& g) {
+void p2g_particle_shader(const orc_scene* s, const Particles<T>& P, uint32_t p, T dt, Grid<T>& g) {
     const T dx = T(s->cell_size);
     const T inv_dx = T(1) / dx;
     const T inv_dx2 = inv_dx * inv_dx;
-    for (uint32_t p = 0; p < P.n; ++p) {
+    {
         V3<T> xp, fx;
         int base[3];
         for (int a = 0; a < 3; ++a) {
@@ -581,11 +584,18 @@ void p2g_shader(const orc_scene* s, const Particles<T>& P, T dt, Grid<T>& g) {
     }
 }
 
+// P2G over all particles in index order.
+template <typename T>
+void p2g_shader(const orc_scene* s, const Particles<T>& P, T dt, Grid<T>& g) {
+    for (uint32_t p = 0; p < P.n; ++p) p2g_particle_shader(s, P, p, dt, g);
+}
+
 // grid_update: mpm.wgsl:1010-1058 (no rigid imprint).
 template <typename T>
-void grid_update_shader(const orc_scene* s, T dt, Grid<T>& g) {
+void grid_update_shader(const orc_scene* s, T dt, Grid<T>& g, bool threads = false) {
     const int n = int(g.G);
     const T bf = T(s->boundary_friction);
+#pragma omp parallel for schedule(static) if (threads)
     for (int iz = 0; iz < n; ++iz)
         for (int iy = 0; iy < n; ++iy)
             for (int ix = 0; ix < n; ++ix) {
@@ -919,6 +929,76 @@ int step_shader(const orc_scene* s, Particles<T>& P, T dt, uint32_t substeps, ui
             uint32_t f = 0;
             for (uint32_t p = 0; p < P.n; ++p)
                 if (!plasticity_shader(s, P, p)) f |= ORC_FAULT_NUMERICAL;
+            fault |= f;
+        }
+    }
+    if (status) {
+        status[0] = fault;
+        status[1] = gathers;
+    }
+    return 0;
+}
+
+// The SHADER step on all host threads: a timing port for the CPU baseline,
+// not an oracle (its sums run in another order than step_shader's).  The P2G
+// scatter runs in two phases over z-slabs of SLAB cells: a particle's stencil
+// spans its base cell and the next two, so with SLAB >= 2 two slabs of the
+// same parity never touch the same node.  The grid update and the particle
+// passes split across threads as they are.  No rigid bodies.
+template <typename T>
+int step_shader_threads(const orc_scene* s, Particles<T>& P, T dt, uint32_t substeps, uint32_t* status) {
+    constexpr int SLAB = 4;
+    Grid<T> g;
+    g.init(s->grid_size);
+    const int G = int(s->grid_size);
+    const int nslab = (G + SLAB - 1) / SLAB;
+    const T inv_dx = T(1) / T(s->cell_size);
+    std::vector<int> slab(P.n);
+    std::vector<uint32_t> order(P.n), start(size_t(nslab) + 1);
+    uint32_t fault = status ? status[0] : 0u;
+    uint32_t gathers = status ? status[1] : 0u;
+    const long long ncell = (long long)g.m.size();
+    for (uint32_t st = 0; st < substeps; ++st) {
+#pragma omp parallel for schedule(static)
+        for (long long c = 0; c < ncell; ++c) {
+            g.m[c] = T(0);
+            for (int a = 0; a < 3; ++a) g.p[3 * c + a] = g.v[3 * c + a] = T(0);
+        }
+        if (fault == 0) {
+#pragma omp parallel for schedule(static)
+            for (long long p = 0; p < (long long)P.n; ++p) {
+                const T xp = (P.x[3 * p + 2] - T(s->origin[2])) * inv_dx;
+                const T bf = std::floor(xp - T(0.5));
+                const int bz = std::isfinite(bf) ? int(std::min(std::max(bf, T(-8)), T(1 << 30))) - s->window[2]
+                                                 : 0;
+                slab[p] = std::min(std::max(bz, 0), G - 1) / SLAB;
+            }
+            std::fill(start.begin(), start.end(), 0u);
+            for (uint32_t p = 0; p < P.n; ++p) start[slab[p] + 1]++;
+            for (int k = 0; k < nslab; ++k) start[k + 1] += start[k];
+            std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+            for (uint32_t p = 0; p < P.n; ++p) order[fill[slab[p]]++] = p;
+            for (int phase = 0; phase < 2; ++phase) {
+#pragma omp parallel for schedule(dynamic, 1)
+                for (int k = phase; k < nslab; k += 2)
+                    for (uint32_t q = start[k]; q < start[k + 1]; ++q) p2g_particle_shader(s, P, order[q], dt, g);
+            }
+        }
+        grid_update_shader(s, dt, g, true);
+        implicit_sweeps(s, dt, g);
+        if (fault == 0) {
+            if (P.n > 0) gathers++;
+            uint32_t f = 0;
+#pragma omp parallel for schedule(static) reduction(| : f)
+            for (long long p = 0; p < (long long)P.n; ++p)
+                if (!g2p_shader(s, P, uint32_t(p), dt, g)) f |= ORC_FAULT_NUMERICAL;
+            fault |= f;
+        }
+        if (fault == 0) {
+            uint32_t f = 0;
+#pragma omp parallel for schedule(static) reduction(| : f)
+            for (long long p = 0; p < (long long)P.n; ++p)
+                if (!plasticity_shader(s, P, uint32_t(p))) f |= ORC_FAULT_NUMERICAL;
             fault |= f;
         }
     }
@@ -1412,6 +1492,17 @@ int orc_step_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, f
     if (s && s->model == ORC_MODEL_CPUDIFF) return -2;
     return step_dispatch<float>(s, n, x, v, C, F, jp, mass, vol, mat, E, dt, substeps, status,
                                 grid_m, grid_p);
+}
+
+// The threaded f32 timing port of the SHADER step (step_shader_threads);
+// *threads receives the thread count used.
+int orc_step_threads_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, float* F, float* jp,
+                         const float* mass, const float* vol, const uint32_t* mat, const float* E, float dt,
+                         uint32_t substeps, uint32_t* status, uint32_t* threads) {
+    if (!s || s->grid_size == 0 || s->model == ORC_MODEL_CPUDIFF) return -2;
+    Particles<float> P{n, x, v, C, F, jp, mass, vol, E, mat};
+    if (threads) *threads = uint32_t(omp_get_max_threads());
+    return step_shader_threads<float>(s, P, dt, substeps, status);
 }
 
 // orc_step_* with rigid bodies (rows: 20 floats per body, see Rigids).

@@ -108,6 +108,13 @@ int orc_step_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, f
                  float* jp, const float* mass, const float* vol, const uint32_t* mat,
                  const float* E, float dt, uint32_t substeps, uint32_t* status, float* grid_m,
                  float* grid_p);
+/* The SHADER step on all host threads (OpenMP), f32: a timing port for the
+ * CPU baseline whose sums run in another order than orc_step_f32's.
+ * *threads receives the thread count. */
+int orc_step_threads_f32(const orc_scene* s, uint32_t n, float* x, float* v, float* C, float* F,
+                         float* jp, const float* mass, const float* vol, const uint32_t* mat,
+                         const float* E, float dt, uint32_t substeps, uint32_t* status,
+                         uint32_t* threads);
 /* P2G only (shader or cpudiff model per s->model), scatter order = particle order. */
 int orc_p2g_f64(const orc_scene* s, uint32_t n, const double* x, const double* v, const double* C,
                 const double* F, const double* jp, const double* mass, const double* vol,

@@ -270,6 +270,21 @@ def test_cpu_diff_model_forward_matches_oracle():
     assert ex < 1e-6 and ev < 1e-3 and eF < 1e-4
 
 
+def test_cpu_diff_floor_row_on_a_rounding_tie():
+    # dx = 1/80 and floor_y = 3 dx: node row 3 sits at 0.0375 in f32 exactly
+    # where floor_y rounds to, but below it in the f64 of the same f32 inputs,
+    # the oracle's arithmetic.  The floor row must be decided the same way,
+    # else a whole layer of nodes changes its boundary rule.
+    cfg, d, meta = workloads.diff_scene(2048, seed=7, grid=80)
+    x, v = d["x"], d["v"]
+    _, g = run_gpu(cfg, x, v, meta["dt"], 32, fields={"youngs": d["E"]})
+    st = oracle_state(cfg, x, v, E=d["E"].astype(np.float64))
+    orc.diff_forward(cfg, st, meta["dt"], 32, record=False)
+    ex, ev = rel_err(g["x"], st.x), rel_err(g["v"], st.v)
+    log(f"  floor tie: x {ex:.2e} v {ev:.2e}")
+    assert ex < 1e-5 and ev < 1e-3
+
+
 def test_escaped_particles_fall_back_to_global_atomics():
     # Fast particles and a long sort interval: stencils leave their sorted
     # 3x3x3 tile neighbourhood, so those particle-substeps take the global

@@ -375,6 +375,35 @@ def test_f32_restatement_tracks_f64():
     assert np.abs(s32.F - s64.F).max() < 1e-3
 
 
+def test_threaded_port_tracks_the_serial_step():
+    # The CPU baseline's threaded f32 port (two-phase z-slab scatter) against
+    # the serial f32 and f64 steps: a mixed elastic/fluid/sand block on the
+    # floor, started partly outside the grid's low z face so that clamped
+    # base cells land in the first slab.
+    cfg = MpmConfig.sand().copy(youngs=1.0e4, poisson=0.2, boundary_friction=0.0,
+                                materials=[Viscous(0, 0), Fluid(1.0e4), Sand()])
+    x = fill_box([-0.3, 0.02, -0.83], [0.3, 0.3, -0.3], 10, 12)
+    assert (x[:, 2] < cfg.grid_origin[2]).any()
+    mat = (np.arange(len(x)) % 3).astype(np.uint32)
+    runs = {}
+    for name, dtype in (("f64", np.float64), ("f32", np.float32), ("threads", np.float32)):
+        st = _state(cfg, x, dtype=dtype)
+        st.mat = mat.copy()
+        if name == "threads":
+            assert orc.step_threads(cfg, st, 1e-3, 40) >= 1
+        else:
+            orc.step(cfg, st, 1e-3, 40)
+        assert st.status[0] == 0 and st.status[1] == 40
+        runs[name] = st
+    for f in ("x", "v", "F"):
+        ref = getattr(runs["f64"], f)
+        e32 = np.abs(getattr(runs["f32"], f) - ref).max()
+        eth = np.abs(getattr(runs["threads"], f) - ref).max()
+        assert eth <= 8 * e32 + 1e-6 * np.abs(ref).max(), (f, eth, e32)
+    with pytest.raises(ValueError):
+        orc.step_threads(cfg, _state(cfg, x), 1e-3, 1)
+
+
 def test_fault_latch_freezes_state():
     cfg = MpmConfig.sand()
     x = fill_box([-0.1, 0.3, -0.1], [0.1, 0.5, 0.1], 4, 2)
