@@ -13,7 +13,10 @@ Prints ONE JSON line.  Besides the headline value it carries:
                 events recorded on the library's stream around every launch
                 inside the timed substeps; `traffic` (HBM bytes per launch) is
                 taken from the FETCH_SIZE/WRITE_SIZE counter summary in
-                profiles/, scaled by the same counters on a copy of known size
+                profiles/, scaled by the same counters on a copy of known size;
+                `roofline.alu` puts the same launches against the VALU issue
+                rate (instructions and flops per launch from the counter
+                summary in profiles/, duration live), the roof that binds
   cpu_baseline  the FP64 single-core oracle (restatement of the reference CPU
                 path) on a bounded sample of the same workload, with the
                 GPU's relative error against it on that sample
@@ -64,6 +67,10 @@ def build_large(sim_cls, scale):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_particles_g2p2g_large_scene_traffic.json")
+ALU_FILE = os.path.join(ROOT, "profiles", "k_particles_g2p2g_large_scene_alu.json")
+# one wave64 VALU instruction per CU per cycle (4 SIMD16 x 4 cycles), 256 CUs
+# at 2.4 GHz; f32 non-packed and f64 FMA issue at this rate
+VALU_ISSUE_PEAK = 256 * 2.4e9
 
 
 def calibrate_copy(mib):
@@ -100,6 +107,27 @@ def measured_traffic(n):
     if t.get("particles") != n:
         return None
     return t.get("bytes_per_launch")
+
+
+def alu_roofline(n, avg_s):
+    """The fused pass against the VALU issue rate: instructions and flops per
+    launch from the committed counter summary (same particle count), over the
+    launch duration measured live."""
+    try:
+        with open(ALU_FILE) as f:
+            a = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if a.get("particles") != n or not avg_s:
+        return None
+    ach = a["valu_instructions_per_launch"] / avg_s
+    return {"bound": "valu_issue", "achieved": ach, "peak": VALU_ISSUE_PEAK,
+            "unit": "wave64 VALU instructions/s", "frac": ach / VALU_ISSUE_PEAK,
+            "valu_instructions_per_launch": a["valu_instructions_per_launch"],
+            "flops_per_particle_substep": {"f32": a["flops_f32_per_launch"] / n, "f64": a["flops_f64_per_launch"] / n},
+            "tflops": {"f32": a["flops_f32_per_launch"] / avg_s / 1e12, "f64": a["flops_f64_per_launch"] / avg_s / 1e12},
+            "lds_bank_conflict_fraction": a["lds_bank_conflict_cycles"] / a["lds_indexed_active_cycles"],
+            "source": a["source"]}
 
 
 def time_steps(sim, dt, k, graph):
@@ -333,7 +361,8 @@ def main():
                 "traffic": traffic, "avg_launch_ms": avg * 1e3, "launches": cnt,
                 "algorithmic_bytes_per_launch": BYTES_FWD * n,
                 "stage_ms_per_substep": {k: (v[0] / args.steps) for k, v in prof.items() if k in STEP_STAGES},
-                "stage_launches": {k: v[1] for k, v in prof.items() if k in STEP_STAGES}}
+                "stage_launches": {k: v[1] for k, v in prof.items() if k in STEP_STAGES},
+                "alu": alu_roofline(n, avg)}
     log(f"profile: {prof}")
     log(f"status: {st}")
 
