@@ -11,7 +11,9 @@ Prints ONE JSON line.  Besides the headline value it carries:
                 algorithmic bytes = 192 B per particle-substep (f32 x, v, C, F
                 read once and written once) and its duration measured with HIP
                 events recorded on the library's stream around every launch
-                inside the timed substeps; `traffic` (HBM bytes per launch) is
+                of a second run of the same number of substeps right after the
+                timed one (the events slow a substep by about 5 %, so `value`
+                is taken without them); `traffic` (HBM bytes per launch) is
                 taken from the FETCH_SIZE/WRITE_SIZE counter summary in
                 profiles/, scaled by the same counters on a copy of known size;
                 `roofline.alu` puts the same launches against the VALU issue
@@ -96,17 +98,15 @@ def calibrate_copy(mib):
     log(f"copy calibration: 3 x {nbytes} B device-to-device")
 
 
-def measured_traffic(n):
-    """HBM bytes per G2P2G launch from the committed counter summary, if it was
-    taken on the same particle count."""
+def traffic_summary(n):
+    """The committed counter and kernel-trace summary of the G2P2G launches,
+    if it was taken on the same particle count."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
-    if t.get("particles") != n:
-        return None
-    return t.get("bytes_per_launch")
+        return {}
+    return t if t.get("particles") == n else {}
 
 
 def alu_roofline(n, avg_s):
@@ -323,6 +323,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay step_n as a hipGraph")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's choice")
+    ap.add_argument("--launch", type=int, nargs=3, metavar=("PARTICLE_BLOCKS", "GRID_BLOCKS", "SORT_EVERY"),
+                    help="fixed launch sizes and sort interval instead of the autotuner's (for profiling runs)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB (for counter scaling)")
     args = ap.parse_args()
@@ -339,27 +342,46 @@ def main():
     log(f"large_scene: {n} particles, grid {meta['grid']}^3, dt {dt}")
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+    # launch sizes and sort interval from the library's autotuner (trial
+    # substeps, state restored), as for the batch
+    if args.launch:
+        sim.set_launch(args.launch[0], args.launch[1])
+        sim.set_sort_interval(args.launch[2])
+        tuned = {"particle_blocks": args.launch[0], "grid_blocks": args.launch[1], "sort_every": args.launch[2],
+                 "fixed": True}
+    else:
+        tuned = None if args.no_tune else sim.autotune(dt, 10)
+    log(f"autotuned: {tuned}")
     if args.warmup:
         time_steps(sim, dt, args.warmup, args.graph)
-    # HIP events around every launch of the timed substeps (graph replays are
-    # one opaque launch, so per-kernel events need the eager path)
-    sim.set_profiling(not args.graph, calls=False)
-    sim.reset_profile()
     el = time_steps(sim, dt, args.steps, args.graph)
-    prof = sim.read_profile()
-    sim.set_profiling(False)
     value = n * args.steps / el
     log(f"timed {args.steps} substeps: {el * 1e3:.2f} ms -> {value:.3e} particle-substeps/s")
+    # The same number of substeps again with HIP events on the library's
+    # stream around every launch, for the per-kernel durations (the events
+    # cost about 5 % of a substep, so `value` comes from the run above; graph
+    # replays are one opaque launch, so per-kernel events need the eager path)
+    sim.set_profiling(not args.graph, calls=False)
+    sim.reset_profile()
+    el_prof = time_steps(sim, dt, args.steps, args.graph)
+    prof = sim.read_profile()
+    sim.set_profiling(False)
+    log(f"profiled {args.steps} substeps: {el_prof * 1e3:.2f} ms")
     st = sim.read_status()
     ms, cnt = prof["g2p2g"]
     avg = ms / max(cnt, 1) * 1e-3
     achieved = BYTES_FWD * n / avg / 1e9 if avg > 0 else None
-    traffic = measured_traffic(n)
+    summary = traffic_summary(n)
+    traffic = summary.get("bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "k_particles<G2P2G>", "achieved": achieved,
                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None,
                 "traffic": traffic, "avg_launch_ms": avg * 1e3, "launches": cnt,
                 "algorithmic_bytes_per_launch": BYTES_FWD * n,
+                "profiled_ms_per_step": el_prof * 1e3 / args.steps,
+                # rocprofv3 kernel-trace mean of the same launches (profiles/): the
+                # event bracket also holds the dispatch delay its markers add
+                "kernel_trace_avg_ms": summary.get("kernel_trace_avg_ms"),
                 "stage_ms_per_substep": {k: (v[0] / args.steps) for k, v in prof.items() if k in STEP_STAGES},
                 "stage_launches": {k: v[1] for k, v in prof.items() if k in STEP_STAGES},
                 "alu": alu_roofline(n, avg)}
@@ -372,7 +394,7 @@ def main():
            "data": "synthetic",
            "config": {"workload": "large_scene", "particles": n, "grid": meta["grid"], "dt": dt,
                       "rho": meta["rho"], "seed": meta["seed"], "fluid_fraction": 0.5,
-                      "graph": bool(args.graph), "fault_bits": st["fault_bits"]},
+                      "graph": bool(args.graph), "autotuned": tuned, "fault_bits": st["fault_bits"]},
            "roofline": roofline}
 
     if not args.no_batched:

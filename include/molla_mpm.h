@@ -163,8 +163,10 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks)
 /* Launch autotuner.  No counterpart in the reference, whose dispatch sizes
  * are fixed; it serves the batched mode, where scene mixes differ widely.
  * Times `trial_substeps` substeps (after a 2-substep warm-up) for candidate
- * persistent particle-pass grids, grid-update grids and sort intervals
- * (coordinate search), applies the fastest as mpm_set_launch /
+ * persistent particle-pass grids and grid-update grids, and two whole sort
+ * intervals (at least `trial_substeps`) for each candidate interval, so
+ * sorting is paid at its own rate (coordinate search, per-substep time);
+ * applies the fastest as mpm_set_launch /
  * mpm_set_sort_interval would, and restores the particle state it started
  * from (stepping resumes from the same state).  Outputs may be NULL. */
 int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,

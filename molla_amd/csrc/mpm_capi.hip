@@ -106,6 +106,7 @@ struct mpm_sim {
     // profiling
     int profiling = 0;         // MPM_PROFILE_* flags
     std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> ev_pool;  // recorded events are reused, never created per launch
     double prof_ms[MPM_STAGE_COUNT] = {0};
     uint32_t prof_n[MPM_STAGE_COUNT] = {0};
     // graphs
@@ -231,6 +232,7 @@ void release(mpm_sim* s) {
         hipEventDestroy(r.a);
         hipEventDestroy(r.b);
     }
+    for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
     void* bufs[] = {s->d_scenes, s->d_mats, s->ps[0], s->ps[1], s->gmp, s->gv, s->tile_flag, s->tile_scene,
                     s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
                     s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
@@ -291,6 +293,21 @@ mpm::GridDev griddev(mpm_sim* s) {
 }
 
 // ---- profiling helpers ----
+// Timing-only events: no system-scope fence when recorded, so recording
+// writes back and invalidates no caches and does not slow the next launch.
+hipError_t timing_event(hipEvent_t* e) { return hipEventCreateWithFlags(e, hipEventDisableSystemFence); }
+
+hipEvent_t take_event(mpm_sim* s) {
+    hipEvent_t e = nullptr;
+    if (!s->ev_pool.empty()) {
+        e = s->ev_pool.back();
+        s->ev_pool.pop_back();
+    } else {
+        timing_event(&e);
+    }
+    return e;
+}
+
 // Events around a launch group (the stages 0-4) or a whole call (5-6) on
 // the handle's stream.  mark_end() places the end event early (the last
 // call wins); otherwise the destructor places it.
@@ -304,8 +321,8 @@ struct Stage {
              (s_->profiling & (stage >= MPM_STAGE_DIFF_FORWARD ? MPM_PROFILE_CALLS : MPM_PROFILE_STAGES))) {
         if (!on) return;
         r.stage = stage;
-        hipEventCreate(&r.a);
-        hipEventCreate(&r.b);
+        r.a = take_event(s);
+        r.b = take_event(s);
         hipEventRecord(r.a, s->stream);
     }
     void mark_end() {
@@ -328,8 +345,8 @@ void flush_profile(mpm_sim* s) {
         hipEventElapsedTime(&ms, r.a, r.b);
         s->prof_ms[r.stage] += ms;
         s->prof_n[r.stage] += 1;
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        s->ev_pool.push_back(r.a);
+        s->ev_pool.push_back(r.b);
     }
     s->prof_pending.clear();
 }
@@ -485,11 +502,11 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = null
 // the last one is observable).
 int grid_update(mpm_sim* s, float dt, bool zero_after, bool react = false) {
     if (s->dry) return MPM_OK;
+    const bool rigid = s->rigid_count > 0;
     Stage st(s, MPM_STAGE_GRID);
     const int nxt = 1 - s->list_cur;
     const mpm::GridDev g = griddev(s);
     uint32_t* cur_count = s->status + mpm::ST_COUNT0 + s->list_cur;
-    const bool rigid = s->rigid_count > 0;
     const mpm::RigidDev rg{s->d_rig, rigid && react ? s->d_wrench : nullptr, s->rigid_count, s->prescribed ? 1u : 0u};
     if (rg.wrench) HIP_TRY(s, mpm::launch_zero_f64(s->d_wrench, 8 * s->rigid_count, s->stream));
     const int parts = s->p2g_mode == MPM_P2G_FUSED ? 1 : 0;
@@ -1218,7 +1235,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
         s->sorted = false;
         return MPM_OK;
     };
-    auto run_trial = [&](uint32_t pb, uint32_t gb, uint32_t se, float& ms) -> int {
+    // ms: per substep over `len` timed substeps after a 2-substep warm-up
+    auto run_trial = [&](uint32_t pb, uint32_t gb, uint32_t se, uint32_t len, float& ms) -> int {
         int r = restore();
         if (r != MPM_OK) return r;
         s->pblocks = pb;
@@ -1226,14 +1244,15 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
         s->sort_every = se;
         if ((r = run_step_n(s, dt, 2)) != MPM_OK) return r;
         HIP_TRY(s, hipEventRecord(ea, s->stream));
-        if ((r = run_step_n(s, dt, trial)) != MPM_OK) return r;
+        if ((r = run_step_n(s, dt, len)) != MPM_OK) return r;
         HIP_TRY(s, hipEventRecord(eb, s->stream));
         HIP_TRY(s, hipEventSynchronize(eb));
         HIP_TRY(s, hipEventElapsedTime(&ms, ea, eb));
+        ms /= float(len);
         return MPM_OK;
     };
     uint32_t best_pb = s->pblocks, best_gb = s->gblocks, best_se = s->sort_every;
-    if (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess) rc = fail(s, MPM_ERR_DEVICE, "event");
+    if (timing_event(&ea) != hipSuccess || timing_event(&eb) != hipSuccess) rc = fail(s, MPM_ERR_DEVICE, "event");
     float best = 1e30f, ms = 0.f;
     const uint32_t pbs[] = {0u, 1024u, 2048u, 4096u, 8192u, 16384u};
     const uint32_t gbs[] = {512u, 1024u, 2048u, 4096u, 8192u};
@@ -1241,7 +1260,7 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     const uint32_t gb0 = best_gb, se0 = best_se ? best_se : 16u;
     for (uint32_t pb : pbs) {
         if (rc != MPM_OK) break;
-        if ((rc = run_trial(pb, gb0, se0, ms)) == MPM_OK && ms < best) {
+        if ((rc = run_trial(pb, gb0, se0, trial, ms)) == MPM_OK && ms < best) {
             best = ms;
             best_pb = pb;
         }
@@ -1249,7 +1268,7 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     best = 1e30f;
     for (uint32_t gb : gbs) {
         if (rc != MPM_OK) break;
-        if ((rc = run_trial(best_pb, gb, se0, ms)) == MPM_OK && ms < best) {
+        if ((rc = run_trial(best_pb, gb, se0, trial, ms)) == MPM_OK && ms < best) {
             best = ms;
             best_gb = gb;
         }
@@ -1257,7 +1276,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     best = 1e30f;
     for (uint32_t se : ses) {
         if (rc != MPM_OK) break;
-        if ((rc = run_trial(best_pb, best_gb, se, ms)) == MPM_OK && ms < best) {
+        // two whole intervals, so each candidate pays its sorts at its own rate
+        if ((rc = run_trial(best_pb, best_gb, se, std::max(trial, 2 * se), ms)) == MPM_OK && ms < best) {
             best = ms;
             best_se = se;
         }
@@ -1583,6 +1603,15 @@ int mpm_set_profiling(mpm_sim* sim, int enabled) {
     if (!sim) return MPM_ERR_INVALID;
     const int all = MPM_PROFILE_STAGES | MPM_PROFILE_CALLS;
     sim->profiling = (enabled & ~all) ? all : enabled;
+    // events for a few hundred profiled substeps up front: creating one per
+    // launch inside a timed region costs more host time than the launch
+    if (sim->profiling & MPM_PROFILE_STAGES) {
+        while (sim->ev_pool.size() < 4096) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(sim, timing_event(&e));
+            sim->ev_pool.push_back(e);
+        }
+    }
     return MPM_OK;
 }
 
