@@ -164,6 +164,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
 #pragma unroll
                     for (int k = 0; k < 9; ++k) M[k] = 0.f;
                     // fetch(i, j, k, g): node (b + (i, j, k)), false when outside the grid
+                    // Row sums over i weighted by w0 (and by i w0), then one
+                    // multiply by w1 w2 per row: 3-6 FMAs per node.
+                    const float wi1 = w[0][1], wi2 = 2.0f * w[0][2];
                     auto gather = [&](auto fetch) {
 #pragma unroll
                         for (int k = 0; k < 3; ++k) {
@@ -175,23 +178,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                                 for (int i = 0; i < 3; ++i) {
                                     float4 g;
                                     if (!fetch(i, j, k, g)) continue;
-                                    const float wt = w[0][i] * wjk;
-                                    const float gx = wt * g.x, gy = wt * g.y, gz = wt * g.z;
-                                    rs[0] += gx;
-                                    rs[1] += gy;
-                                    rs[2] += gz;
+                                    rs[0] += w[0][i] * g.x;
+                                    rs[1] += w[0][i] * g.y;
+                                    rs[2] += w[0][i] * g.z;
                                     if (i > 0) {
-                                        ri[0] += float(i) * gx;
-                                        ri[1] += float(i) * gy;
-                                        ri[2] += float(i) * gz;
+                                        const float wi = i == 1 ? wi1 : wi2;
+                                        ri[0] += wi * g.x;
+                                        ri[1] += wi * g.y;
+                                        ri[2] += wi * g.z;
                                     }
                                 }
+                                const float wj = float(j) * wjk, wk = float(k) * wjk;
 #pragma unroll
                                 for (int r = 0; r < 3; ++r) {
-                                    vn[r] += rs[r];
-                                    M[3 * r] += ri[r];
-                                    M[3 * r + 1] += float(j) * rs[r];
-                                    M[3 * r + 2] += float(k) * rs[r];
+                                    vn[r] += wjk * rs[r];
+                                    M[3 * r] += wjk * ri[r];
+                                    M[3 * r + 1] += wj * rs[r];
+                                    M[3 * r + 2] += wk * rs[r];
                                 }
                             }
                         }

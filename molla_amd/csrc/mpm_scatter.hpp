@@ -345,13 +345,21 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             else nodes(std::true_type{});
         }
         __syncthreads();
-        const double im = 1.0 / sm64, ip = 1.0 / sp64;
         float4* part = a.part + (size_t)chunk * LDS_CELLS;
-        for (int i = tid; i < vol; i += BLOCK) {
-            using acc_t = typename std::conditional<WIDE, long long, int>::type;
-            part[i] = make_float4(MASS ? (float)((double)(acc_t)L.acc[0][i] * im) : 0.f,
-                                  (float)((double)(acc_t)L.acc[1][i] * ip), (float)((double)(acc_t)L.acc[2][i] * ip),
-                                  (float)((double)(acc_t)L.acc[3][i] * ip));
+        if constexpr (WIDE) {
+            // int64 sums: exact in f64 up to 2^53, one rounding to f32 after the scale
+            const double im = 1.0 / sm64, ip = 1.0 / sp64;
+            for (int i = tid; i < vol; i += BLOCK)
+                part[i] = make_float4(MASS ? (float)((double)(long long)L.acc[0][i] * im) : 0.f,
+                                      (float)((double)(long long)L.acc[1][i] * ip),
+                                      (float)((double)(long long)L.acc[2][i] * ip),
+                                      (float)((double)(long long)L.acc[3][i] * ip));
+        } else {
+            // int32 sums: one rounding to f32, then an exact power-of-two scale
+            const float im = (float)(1.0 / sm64), ip = (float)(1.0 / sp64);
+            for (int i = tid; i < vol; i += BLOCK)
+                part[i] = make_float4(MASS ? (float)(int)L.acc[0][i] * im : 0.f, (float)(int)L.acc[1][i] * ip,
+                                      (float)(int)L.acc[2][i] * ip, (float)(int)L.acc[3][i] * ip);
         }
         __syncthreads();
     }
