@@ -122,6 +122,7 @@ struct mpm_sim {
     int diff_ps = 0;
     float* tape = nullptr;     // pre-states of one checkpoint segment (NCOMP planes each)
     float4* gtape = nullptr;   // per substep of the segment: grid v, then P2G total (m, p)
+    float* atape = nullptr;    // per substep of the segment: the P2G affine A (9 planes)
     uint32_t tape_len = 0;
     int tape_seg = -1;         // the segment the tape holds from the forward itself (-1: none)
     float* adj = nullptr;      // NADJ adjoint planes
@@ -241,7 +242,7 @@ void release(mpm_sim* s) {
     for (void* b : bufs)
         if (b) hipFree(b);
     for (float* c : s->ckpt) hipFree(c);
-    void* dbufs[] = {s->tape, s->gtape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap, s->d_rig, s->d_wrench};
+    void* dbufs[] = {s->tape, s->gtape, s->atape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap, s->d_rig, s->d_wrench};
     for (void* b : dbufs)
         if (b) hipFree(b);
     if (s->hpin) hipHostFree(s->hpin);
@@ -476,8 +477,9 @@ void use_sorted_chunks(mpm_sim* s, mpm::StepArgs& a, int& blocks) {
 }
 
 // `ps` (optional) runs the pass on another set of state planes of the same
-// slot order (the adjoint's recomputed pre-states).
-int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = nullptr) {
+// slot order (the adjoint's recomputed pre-states); `aff_out` (optional)
+// receives the scattered particles' P2G affine (the diff tape).
+int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = nullptr, float* aff_out = nullptr) {
     // a scattering pass over the sorted chunks leaves their boxes for the
     // next pass's gather; a plain G2P moves the particles away from them
     const bool sorted_chunks = s->p2g_mode == MPM_P2G_FUSED && s->sorted;
@@ -489,6 +491,7 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = null
     Stage st(s, mode == mpm::MODE_P2G ? MPM_STAGE_P2G : (mode == mpm::MODE_G2P ? MPM_STAGE_G2P : MPM_STAGE_G2P2G));
     mpm::StepArgs a = args(s, dt, list_out);
     if (ps) a.ps = a.ps_out = ps;
+    a.aff_out = aff_out;
     int blocks = particle_blocks(s);
     if (sorted_chunks) use_sorted_chunks(s, a, blocks);
     else a.box_gather = 0;
@@ -794,11 +797,14 @@ int ensure_tape(mpm_sim* s, uint32_t every) {
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     if (s->tape) HIP_TRY(s, hipFree(s->tape));
     if (s->gtape) HIP_TRY(s, hipFree(s->gtape));
+    if (s->atape) HIP_TRY(s, hipFree(s->atape));
     s->tape = nullptr;
     s->gtape = nullptr;
+    s->atape = nullptr;
     s->tape_len = 0;
     HIP_TRY(s, dalloc(&s->tape, (size_t)every * mpm::NCOMP * s->cap));
     HIP_TRY(s, dalloc(&s->gtape, (size_t)every * 2 * cells));
+    HIP_TRY(s, dalloc(&s->atape, (size_t)every * 9 * s->cap));
     s->tape_len = every;
     return MPM_OK;
 }
@@ -819,7 +825,7 @@ int record_segment(mpm_sim* s, float dt, uint32_t len) {
     const size_t cells = (size_t)s->total_tiles * 64;
     float* live = s->ps[s->ps_cur];
     HIP_TRY(s, hipMemcpyAsync(s->tape, live, (size_t)24 * s->cap * 4, hipMemcpyDeviceToDevice, s->stream));
-    if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur, s->tape)) != MPM_OK) return rc;
+    if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur, s->tape, s->atape)) != MPM_OK) return rc;
     const mpm::GridDev g = griddev(s);
     const int gb = grid_blocks(s);
     for (uint32_t k = 0; k < len; ++k) {
@@ -834,6 +840,7 @@ int record_segment(mpm_sim* s, float dt, uint32_t len) {
         a.ps = s->tape + k * st_words;
         a.ps_out = last ? live : a.ps + st_words;
         a.gv = gk.gv;
+        if (!last) a.aff_out = s->atape + (size_t)(k + 1) * 9 * s->cap;  // substep k+1's P2G
         int blocks = 0;
         use_sorted_chunks(s, a, blocks);
         HIP_TRY(s, mpm::launch_particle_pass(last ? mpm::MODE_G2P : mpm::MODE_G2P2G, int(s->sh[0].model), a, 1,
@@ -974,7 +981,7 @@ int recompute_segment(mpm_sim* s, uint32_t seg, uint32_t len, bool checked, uint
     }
     float* t0 = s->tape;
     HIP_TRY(s, hipMemcpyAsync(t0, s->ckpt[seg], (size_t)24 * s->cap * 4, hipMemcpyDeviceToDevice, s->stream));
-    int rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0);
+    int rc = particle_pass(s, mpm::MODE_P2G, dt, L, t0, s->atape);
     if (rc != MPM_OK) return rc;
     for (uint32_t k = 0; k < len; ++k) {
         float* tk = s->tape + k * st_words;
@@ -988,6 +995,7 @@ int recompute_segment(mpm_sim* s, uint32_t seg, uint32_t len, bool checked, uint
         a.ps_out = tk + st_words;  // state k+1 goes straight to the next tape slot
         a.gv = gk.gv;
         a.write_vc = 1;
+        a.aff_out = s->atape + (size_t)(k + 1) * 9 * s->cap;  // and its P2G affine
         int blocks = 0;
         use_sorted_chunks(s, a, blocks);
         HIP_TRY(s, mpm::launch_particle_pass(mpm::MODE_G2P2G, int(s->sh[0].model), a, 1, blocks, s->stream));
@@ -1058,6 +1066,7 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
         const int R = 1 - L;
         auto tape_at = [&](int k) { return s->tape + (size_t)k * st_words; };
         auto gv_at = [&](int k) { return s->gtape + (size_t)2 * k * cells; };
+        auto aff_at = [&](int k) { return s->atape + (size_t)k * 9 * cap; };
         if (fused) {
             int blocks = 0;
             mpm::AdjArgs q = adj_args(s, dt, R, tape_at(int(len) - 1), blocks);
@@ -1069,6 +1078,7 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
                 HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 0, gb,
                                                 s->stream));
                 mpm::AdjArgs qk = adj_args(s, dt, R, tape_at(k), blocks);
+                qk.aff = aff_at(k);
                 if (k > 0)
                     HIP_TRY(s, mpm::launch_adj_fused(qk, tape_at(k - 1), gv_at(k - 1), blocks, s->stream));
                 else
@@ -1086,6 +1096,7 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
             int blocks = 0;
             mpm::AdjArgs q = adj_args(s, dt, R, tk, blocks);
             q.a.gv = gvk;
+            q.aff = aff_at(k);
             HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
             mpm::GridDev gk = g;
             gk.gmp = gvk + cells;

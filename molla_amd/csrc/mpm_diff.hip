@@ -156,63 +156,30 @@ __device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneD
 }
 
 // Reverse of P2G, of the stress and of the SVD for the particle in `slot`
-// of the pre-state a.ps, gathering (dJ/dm, dJ/dp) from gadj
+// of the pre-state a.ps, with the P2G affine A the forward scattered (the
+// planes `aff`), gathering (dJ/dm, dJ/dp) from gadj
 // (mpm_adjoint.rs:389-487, mpm_cpu_diff.rs:330-379, dense3.rs:321-372).  On
 // entry gx, gF hold the reverse G2P's output; on return gx, gv, gC, gF are
 // dJ/d(pre-state), gE has dJ/dE_p added, and (amu, alam) are the particle's
 // dJ/dmu, dJ/dlambda.
 __device__ __forceinline__ void adj_p2g_particle(const StepArgs& a, const float4* __restrict__ gadj,
-                                                 const SceneDev& sc, uint32_t slot, float* gx, float* gF, float* gv,
-                                                 float* gC, float& gE, double& amu, double& alam) {
+                                                 const float* __restrict__ aff, const SceneDev& sc, uint32_t slot,
+                                                 float* gx, float* gF, float* gv, float* gC, float& gE, double& amu,
+                                                 double& alam) {
     const uint32_t cap = a.cap;
     const float* __restrict__ ps = a.ps;
     const float dt = a.dt;
-    float x[3], v[3], C[9], F[9];
+    float x[3], v[3], A[9];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         x[k] = ps[(size_t)(CX + k) * cap + slot];
         v[k] = ps[(size_t)(CV + k) * cap + slot];
     }
+    // A = m C + 4/dx^2 stress (stress = -dt vol P F^T) as the forward scattered
+    // it: the tape holds it, so the FP64 SVD is not live across the gather
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        C[k] = ps[(size_t)(CC + k) * cap + slot];
-        F[k] = ps[(size_t)(CF + k) * cap + slot];
-    }
+    for (int k = 0; k < 9; ++k) A[k] = aff[(size_t)k * cap + slot];
     const float m = ps[(size_t)CMASS * cap + slot];
-    const float vol = ps[(size_t)CVOL * cap + slot];
-    float mu = sc.mu, lam = sc.lam;
-    if (a.has_E) {
-        const float E = ps[(size_t)CE * cap + slot];
-        mu = E / (2.0f * (1.0f + sc.poisson));
-        lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
-    }
-    // corotated-SVK PK1 from the FP64 SVD (mpm_cpu_diff.rs:279-287)
-    double U[9], s[3], V[9];
-    svd3_f64_raw(F, U, s, V);
-    svd3_signs<MODEL_CPUDIFF>(U, s, V);
-    const double dmu = mu, dlam = lam;
-    const double trsi = s[0] + s[1] + s[2] - 3.0;
-    Dm3 P;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double r = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
-            const double fr = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
-                              U[3 * i + 2] * s[2] * V[3 * j + 2];
-            P(i, j) = 2.0 * dmu * (fr - r) + dlam * trsi * r;
-        }
-    // A = m C + 4/dx^2 stress, stress = -dt vol P F^T
-    const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
-    const double coef = -(double)dt * (double)vol;
-    float A[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const double st = coef * (P(i, 0) * F[3 * j] + P(i, 1) * F[3 * j + 1] + P(i, 2) * F[3 * j + 2]);
-            A[3 * i + j] = m * C[3 * i + j] + s4 * (float)st;
-        }
     float mv[3] = {m * v[0], m * v[1], m * v[2]};
     float xp[3], fx[3], w[3][3], dw[3][3];
     int b[3];
@@ -265,6 +232,34 @@ __device__ __forceinline__ void adj_p2g_particle(const StepArgs& a, const float4
         gx[k] += gpos[k];
         gv[k] = m * gw[k];
     }
+    // corotated-SVK PK1 from the FP64 SVD (mpm_cpu_diff.rs:279-287)
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = ps[(size_t)(CF + k) * cap + slot];
+    const float vol = ps[(size_t)CVOL * cap + slot];
+    float mu = sc.mu, lam = sc.lam;
+    if (a.has_E) {
+        const float E = ps[(size_t)CE * cap + slot];
+        mu = E / (2.0f * (1.0f + sc.poisson));
+        lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
+    }
+    double U[9], s[3], V[9];
+    svd3_f64_raw(F, U, s, V);
+    svd3_signs<MODEL_CPUDIFF>(U, s, V);
+    const double dmu = mu, dlam = lam;
+    const double trsi = s[0] + s[1] + s[2] - 3.0;
+    Dm3 P;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double r = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
+            const double fr = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                              U[3 * i + 2] * s[2] * V[3 * j + 2];
+            P(i, j) = 2.0 * dmu * (fr - r) + dlam * trsi * r;
+        }
+    const float s4 = 4.0f * sc.inv_dx * sc.inv_dx;
+    const double coef = -(double)dt * (double)vol;
 #pragma unroll
     for (int k = 0; k < 9; ++k) gC[k] = m * T[k];
     // stress = coef P F^T with dJ/dstress = s4 T:
@@ -475,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 #pragma unroll
             for (int k = 0; k < 9; ++k) gF[k] = adj[(size_t)(AF + k) * cap + slot];
             float gE = adj[(size_t)AE * cap + slot];
-            adj_p2g_particle(a, q.gadj, sc, slot, gx, gF, gv, gC, gE, amu, alam);
+            adj_p2g_particle(a, q.gadj, q.aff, sc, slot, gx, gF, gv, gC, gE, amu, alam);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 adj[(size_t)(AX + k) * cap + slot] = gx[k];
@@ -520,7 +515,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) v
 #pragma unroll
             for (int k = 0; k < 9; ++k) gF[k] = adj[(size_t)(AF + k) * cap + slot];
             float gE = adj[(size_t)AE * cap + slot];
-            adj_p2g_particle(a, q.gadj, sc, slot, gx, gF, gv, gC, gE, amu, alam);
+            adj_p2g_particle(a, q.gadj, q.aff, sc, slot, gx, gF, gv, gC, gE, amu, alam);
             adj[(size_t)AE * cap + slot] = gE;
         }
         add_chunk_mu_lambda(q.gml, ci, amu, alam, red);

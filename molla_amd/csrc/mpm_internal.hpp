@@ -135,6 +135,8 @@ struct StepArgs {
     int has_E;     // per-particle Young's modulus plane is valid
     int box_gather;  // 1: chunk_box holds, per chunk of this pass, the P2G box of
                      // the current positions (the previous pass scattered them)
+    float* aff_out;  // or null: 9 planes of a.cap floats that receive each scattered
+                     // particle's P2G affine A = m C + stress term (the diff tape)
 };
 
 struct RigidDev;  // rigid bodies on the grid (mpm_rigid.hpp)
@@ -208,6 +210,7 @@ struct AdjArgs {
     float* adj;    // NADJ planes of a.cap floats
     float4* gadj;  // per cell: (dJ/dm, dJ/dp) after the grid reverse
     double* gml;   // per chunk: accumulated (dJ/dmu, dJ/dlambda)
+    const float* aff;  // the substep's P2G affine A (9 planes), recorded by its forward
 };
 // Reverse G2P + x/F advance: gathers grid v, scatters dJ/d(grid v) (partials),
 // folds the G2P part of dJ/dx into adj x and replaces adj F by (I + dt C)^T adj F.

@@ -131,6 +131,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                     Svd32 d;
                     svd3<MODEL>(F, d);
                     affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp, dt, vol, m, C, sp.A);
+                    if (a.aff_out)
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) a.aff_out[(size_t)k * cap + slot] = sp.A[k];
                 }
             } else {
                 // ---- G2P (mpm.wgsl:1241-1353 / mpm_cpu_diff.rs:544-580) ----
@@ -375,6 +378,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                             if (MODEL == MODEL_CPUDIFF) svd3<MODEL_CPUDIFF>(Fn, d);
                             (void)have_svd;
                             affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp_new, dt, vol, m, Cn, sp.A);
+                            if (a.aff_out)
+#pragma unroll
+                                for (int k = 0; k < 9; ++k) a.aff_out[(size_t)k * cap + slot] = sp.A[k];
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
                                 x[k] = nx[k];
