@@ -97,39 +97,66 @@ __device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneD
         dw[k][2] = fx[k] - 0.5f;
     }
     // G2P reverse (mpm_adjoint.rs:249-305): with B = 4/dx^2 dJ/dC',
-    //   dJ/dx += sum grad(w) (v_i . (dJ/dv' + B d)) - B^T sum w v_i
+    //   dJ/dx += sum grad(w) (v_i . (dJ/dv' + B d)) - B^T sum w v_i,
+    // summed row by row (fixed j, k): with d = ((i, j, k) - fx) dx the node
+    // term is u_jk + i b0 (b0 = dx B[:, 0]); with Sw = sum_i w0 h and
+    // Sdw = sum_i dw0 h the gradient sum is
+    //   idx (sum w1 w2 Sdw, sum dw1 w2 Sw, sum w1 dw2 Sw),
+    // and C' = 4/dx^2 sum w v_i d^T = 4/dx (Mv - vn fx^T), Mv = sum w v_i (i, j, k)^T.
     const float dx = sc.dx, idx = sc.inv_dx;
-    float vn[3] = {0.f, 0.f, 0.f}, Cn[9], acc[3] = {0.f, 0.f, 0.f};
+    float b0[3], u0[3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Cn[k] = 0.f;
+    for (int r = 0; r < 3; ++r) {
+        b0[r] = dx * B[3 * r];
+        u0[r] = gvp[r] - (b0[r] * fx[0] + dx * B[3 * r + 1] * fx[1] + dx * B[3 * r + 2] * fx[2]);
+    }
+    const float wi[3] = {0.f, w[0][1], 2.0f * w[0][2]};
+    float vn[3] = {0.f, 0.f, 0.f}, Mv[9], acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Mv[k] = 0.f;
     for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j) {
+            const float wjk = w[1][j] * w[2][k];
+            float u[3], rs[3] = {0.f, 0.f, 0.f}, ri[3] = {0.f, 0.f, 0.f}, Sw = 0.f, Sdw = 0.f;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) u[r] = u0[r] + float(j) * (dx * B[3 * r + 1]) + float(k) * (dx * B[3 * r + 2]);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
                 if (!in_grid(sc, ix, iy, iz)) continue;
                 const float4 g = a.gv[cell_index(sc, ix, iy, iz)];
-                const float wt = w[0][i] * w[1][j] * w[2][k];
-                const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
-                            d2 = (float(k) - fx[2]) * dx;
-                vn[0] += wt * g.x;
-                vn[1] += wt * g.y;
-                vn[2] += wt * g.z;
-                const float sw = s4 * wt;
-                const float gx0 = sw * g.x, gy0 = sw * g.y, gz0 = sw * g.z;
-                Cn[0] += gx0 * d0; Cn[1] += gx0 * d1; Cn[2] += gx0 * d2;
-                Cn[3] += gy0 * d0; Cn[4] += gy0 * d1; Cn[5] += gy0 * d2;
-                Cn[6] += gz0 * d0; Cn[7] += gz0 * d1; Cn[8] += gz0 * d2;
-                const float h = g.x * (gvp[0] + B[0] * d0 + B[1] * d1 + B[2] * d2) +
-                                g.y * (gvp[1] + B[3] * d0 + B[4] * d1 + B[5] * d2) +
-                                g.z * (gvp[2] + B[6] * d0 + B[7] * d1 + B[8] * d2);
-                acc[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
-                acc[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
-                acc[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
+                const float h = g.x * (u[0] + float(i) * b0[0]) + g.y * (u[1] + float(i) * b0[1]) +
+                                g.z * (u[2] + float(i) * b0[2]);
+                Sw += w[0][i] * h;
+                Sdw += dw[0][i] * h;
+                rs[0] += w[0][i] * g.x;
+                rs[1] += w[0][i] * g.y;
+                rs[2] += w[0][i] * g.z;
+                if (i > 0) {
+                    ri[0] += wi[i] * g.x;
+                    ri[1] += wi[i] * g.y;
+                    ri[2] += wi[i] * g.z;
+                }
             }
 #pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                vn[r] += wjk * rs[r];
+                Mv[3 * r] += wjk * ri[r];
+                Mv[3 * r + 1] += float(j) * wjk * rs[r];
+                Mv[3 * r + 2] += float(k) * wjk * rs[r];
+            }
+            acc[0] += wjk * Sdw;
+            acc[1] += dw[1][j] * w[2][k] * Sw;
+            acc[2] += w[1][j] * dw[2][k] * Sw;
+        }
+    float Cn[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Cn[3 * r + c] = s4 * dx * (Mv[3 * r + c] - vn[r] * fx[c]);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        acc[k] -= B[k] * vn[0] + B[3 + k] * vn[1] + B[6 + k] * vn[2];
+        acc[k] = acc[k] * idx - (B[k] * vn[0] + B[3 + k] * vn[1] + B[6 + k] * vn[2]);
         gx[k] += acc[k];
     }
     // dJ/dF (pre) = (I + dt C')^T dJ/dF'
@@ -195,40 +222,64 @@ __device__ __forceinline__ void adj_p2g_particle(const StepArgs& a, const float4
         dw[k][2] = fx[k] - 0.5f;
     }
     // P2G reverse (mpm_adjoint.rs:389-487): with gw = sum w dJ/dp_i and
-    // T = sum w dJ/dp_i d^T:  dJ/dv = m gw, dJ/dC = m T,
-    // dJ/dstress = 4/dx^2 T, dJ/dx = sum grad(w) (dJ/dp_i . (m v + A d)
-    // + m dJ/dm_i) - A^T gw
+    // T = sum w dJ/dp_i d^T:  dJ/dv = m gw, dJ/dC = m T, dJ/dstress = 4/dx^2 T,
+    // dJ/dx = sum grad(w) (dJ/dp_i . (m v + A d) + m dJ/dm_i) - A^T gw,
+    // summed row by row as in the reverse G2P (A d = u_jk - m v + i a0,
+    // a0 = dx A[:, 0]) with T = dx (Mt - gw fx^T), Mt = sum w dJ/dp_i (i, j, k)^T.
     const float dx = sc.dx, idx = sc.inv_dx;
-    float gw[3] = {0.f, 0.f, 0.f}, T[9], gpos[3] = {0.f, 0.f, 0.f};
+    float a0[3], u0[3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) T[k] = 0.f;
+    for (int r = 0; r < 3; ++r) {
+        a0[r] = dx * A[3 * r];
+        u0[r] = mv[r] - (a0[r] * fx[0] + dx * A[3 * r + 1] * fx[1] + dx * A[3 * r + 2] * fx[2]);
+    }
+    const float wi[3] = {0.f, w[0][1], 2.0f * w[0][2]};
+    float gw[3] = {0.f, 0.f, 0.f}, Mt[9], gpos[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Mt[k] = 0.f;
     for (int k = 0; k < 3; ++k)
-        for (int j = 0; j < 3; ++j)
+        for (int j = 0; j < 3; ++j) {
+            const float wjk = w[1][j] * w[2][k];
+            float u[3], rs[3] = {0.f, 0.f, 0.f}, ri[3] = {0.f, 0.f, 0.f}, Sw = 0.f, Sdw = 0.f;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) u[r] = u0[r] + float(j) * (dx * A[3 * r + 1]) + float(k) * (dx * A[3 * r + 2]);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
                 if (!in_grid(sc, ix, iy, iz)) continue;
                 const float4 g = gadj[cell_index(sc, ix, iy, iz)];
-                const float wt = w[0][i] * w[1][j] * w[2][k];
-                const float d0 = (float(i) - fx[0]) * dx, d1 = (float(j) - fx[1]) * dx,
-                            d2 = (float(k) - fx[2]) * dx;
-                gw[0] += wt * g.y;
-                gw[1] += wt * g.z;
-                gw[2] += wt * g.w;
-                const float ax = wt * g.y, ay = wt * g.z, az = wt * g.w;
-                T[0] += ax * d0; T[1] += ax * d1; T[2] += ax * d2;
-                T[3] += ay * d0; T[4] += ay * d1; T[5] += ay * d2;
-                T[6] += az * d0; T[7] += az * d1; T[8] += az * d2;
-                const float h = g.y * (mv[0] + A[0] * d0 + A[1] * d1 + A[2] * d2) +
-                                g.z * (mv[1] + A[3] * d0 + A[4] * d1 + A[5] * d2) +
-                                g.w * (mv[2] + A[6] * d0 + A[7] * d1 + A[8] * d2) + m * g.x;
-                gpos[0] += dw[0][i] * w[1][j] * w[2][k] * idx * h;
-                gpos[1] += w[0][i] * dw[1][j] * w[2][k] * idx * h;
-                gpos[2] += w[0][i] * w[1][j] * dw[2][k] * idx * h;
+                const float h = m * g.x + g.y * (u[0] + float(i) * a0[0]) + g.z * (u[1] + float(i) * a0[1]) +
+                                g.w * (u[2] + float(i) * a0[2]);
+                Sw += w[0][i] * h;
+                Sdw += dw[0][i] * h;
+                rs[0] += w[0][i] * g.y;
+                rs[1] += w[0][i] * g.z;
+                rs[2] += w[0][i] * g.w;
+                if (i > 0) {
+                    ri[0] += wi[i] * g.y;
+                    ri[1] += wi[i] * g.z;
+                    ri[2] += wi[i] * g.w;
+                }
             }
 #pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                gw[r] += wjk * rs[r];
+                Mt[3 * r] += wjk * ri[r];
+                Mt[3 * r + 1] += float(j) * wjk * rs[r];
+                Mt[3 * r + 2] += float(k) * wjk * rs[r];
+            }
+            gpos[0] += wjk * Sdw;
+            gpos[1] += dw[1][j] * w[2][k] * Sw;
+            gpos[2] += w[1][j] * dw[2][k] * Sw;
+        }
+    float T[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) T[3 * r + c] = dx * (Mt[3 * r + c] - gw[r] * fx[c]);
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        gpos[k] -= A[k] * gw[0] + A[3 + k] * gw[1] + A[6 + k] * gw[2];
+        gpos[k] = gpos[k] * idx - (A[k] * gw[0] + A[3 + k] * gw[1] + A[6 + k] * gw[2]);
         gx[k] += gpos[k];
         gv[k] = m * gw[k];
     }
