@@ -212,7 +212,7 @@ std::string validate(const mpm_scene_config& c) {
 }
 
 void drop_graphs(mpm_sim* s) {
-    for (auto& kv : s->graphs) hipGraphExecDestroy(kv.second);
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
 }
 
@@ -225,28 +225,28 @@ hipError_t dalloc(T** p, size_t count) {
 
 void release(mpm_sim* s) {
     if (!s) return;
-    hipSetDevice(s->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
-    for (auto& kv : s->graphs) hipGraphExecDestroy(kv.second);
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
     s->graphs.clear();
     for (auto& r : s->prof_pending) {
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
     }
-    for (hipEvent_t e : s->ev_pool) hipEventDestroy(e);
+    for (hipEvent_t e : s->ev_pool) (void)hipEventDestroy(e);
     void* bufs[] = {s->d_scenes, s->d_mats, s->ps[0], s->ps[1], s->gmp, s->gv, s->tile_flag, s->tile_scene,
                     s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
                     s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
                     s->src_slot, s->chunk_box, s->part, s->schunks, s->chunk_of_slot, s->cflags,
                     s->cexcl, s->scan_scratch, s->smark, s->spos, s->sscan};
     for (void* b : bufs)
-        if (b) hipFree(b);
-    for (float* c : s->ckpt) hipFree(c);
+        if (b) (void)hipFree(b);
+    for (float* c : s->ckpt) (void)hipFree(c);
     void* dbufs[] = {s->tape, s->gtape, s->atape, s->adj, s->gadj, s->gml, s->gv2, s->ck_aux, s->dsnap, s->d_rig, s->d_wrench};
     for (void* b : dbufs)
-        if (b) hipFree(b);
-    if (s->hpin) hipHostFree(s->hpin);
-    if (s->stream) hipStreamDestroy(s->stream);
+        if (b) (void)hipFree(b);
+    if (s->hpin) (void)hipHostFree(s->hpin);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
 mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
@@ -304,7 +304,7 @@ hipEvent_t take_event(mpm_sim* s) {
         e = s->ev_pool.back();
         s->ev_pool.pop_back();
     } else {
-        timing_event(&e);
+        (void)timing_event(&e);
     }
     return e;
 }
@@ -324,28 +324,29 @@ struct Stage {
         r.stage = stage;
         r.a = take_event(s);
         r.b = take_event(s);
-        hipEventRecord(r.a, s->stream);
+        (void)hipEventRecord(r.a, s->stream);
     }
     void mark_end() {
         if (!on) return;
-        hipEventRecord(r.b, s->stream);
+        (void)hipEventRecord(r.b, s->stream);
         ended = true;
     }
     ~Stage() {
         if (!on) return;
-        if (!ended) hipEventRecord(r.b, s->stream);
+        if (!ended) (void)hipEventRecord(r.b, s->stream);
         s->prof_pending.push_back(r);
     }
 };
 
 void flush_profile(mpm_sim* s) {
     if (s->prof_pending.empty()) return;
-    hipStreamSynchronize(s->stream);
+    (void)hipStreamSynchronize(s->stream);
     for (auto& r : s->prof_pending) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, r.a, r.b);
-        s->prof_ms[r.stage] += ms;
-        s->prof_n[r.stage] += 1;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {  // an unrecorded pair is not counted
+            s->prof_ms[r.stage] += ms;
+            s->prof_n[r.stage] += 1;
+        }
         s->ev_pool.push_back(r.a);
         s->ev_pool.push_back(r.b);
     }
@@ -1295,8 +1296,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     }
     int rr = restore();
     if (rc == MPM_OK) rc = rr;
-    if (ea) hipEventDestroy(ea);
-    if (eb) hipEventDestroy(eb);
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
     s->profiling = prof;
     s->pblocks = best_pb;
     s->gblocks = best_gb;
@@ -1415,7 +1416,7 @@ int mpm_set_scene_params(mpm_sim* sim, uint32_t scene, const mpm_scene_config* c
     SceneDev d = make_scene(*cfg, sim->sh[scene].tile_base, sim->sh[scene].mat_base);
     sim->cfg[scene] = *cfg;
     sim->sh[scene] = d;
-    for (auto& kv : sim->graphs) hipGraphExecDestroy(kv.second);
+    for (auto& kv : sim->graphs) (void)hipGraphExecDestroy(kv.second);
     sim->graphs.clear();
     sim->sorted = false;
     return upload_scenes(sim);
@@ -1441,7 +1442,7 @@ int mpm_set_time_step(mpm_sim* sim, uint32_t jacobi_iters) {
     if (jacobi_iters > 0 && sim->gmp && !sim->gv2)
         HIP_TRY(sim, dalloc(&sim->gv2, (size_t)sim->total_tiles * 64));
     sim->jacobi_iters = jacobi_iters;
-    for (auto& kv : sim->graphs) hipGraphExecDestroy(kv.second);
+    for (auto& kv : sim->graphs) (void)hipGraphExecDestroy(kv.second);
     sim->graphs.clear();
     return MPM_OK;
 }
@@ -1515,7 +1516,7 @@ int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
         if (e != hipSuccess) return hip_fail(sim, e, "hipStreamEndCapture");
         hipGraphExec_t ex;
         e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e != hipSuccess) return hip_fail(sim, e, "hipGraphInstantiate");
         it = sim->graphs.emplace(key, ex).first;
     } else {
@@ -1550,7 +1551,7 @@ int mpm_read_grid_mass_momentum(mpm_sim* sim, uint32_t scene, float* mass, float
     if (e == hipSuccess) e = hipMemcpyAsync(mass, buf, cells * 4, hipMemcpyDeviceToHost, sim->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(momentum, buf + cells, cells * 12, hipMemcpyDeviceToHost, sim->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
-    hipFree(buf);
+    (void)hipFree(buf);
     if (e != hipSuccess) return hip_fail(sim, e, "read_grid_mass_momentum");
     return MPM_OK;
 }
@@ -1811,7 +1812,7 @@ int soil_fail(mpm_sim* s) { return fail(s, MPM_ERR_INVALID, mpm_soil_last_error(
 struct TempWords {
     uint32_t* p = nullptr;
     ~TempWords() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 
