@@ -187,6 +187,7 @@ def run_batched(steps, warmup, scale, autotune=True):
     sim.close()
     return {"value": n * steps / el, "unit": "particle-substeps/s", "scenes": n_scenes,
             "particles": n, "substeps": steps, "ms_per_step": 1e3 * el / steps,
+            "hbm_frac": n * steps / el * BYTES_FWD / (HBM_PEAK_GBS * 1e9),
             "grids": [meta["grid_min"], meta["grid_max"]], "graph": True, "autotuned": tuned}
 
 
@@ -200,11 +201,15 @@ def run_plain(scale):
     cfg, x, meta = workloads.elastic_cube(n, seed=1)
     sim = MpmSimulation(cfg, max_particles=n)
     sim.set_particles(x, None)
+    # the autotuner's launch sizes, as for the batch; eager stepping (a graph
+    # replay of this two-kernel substep is slower on this stack)
+    tuned = sim.autotune(meta["dt"], 20)
     sim.step_n(meta["dt"], 20)
     el = time_steps(sim, meta["dt"], 1000, graph=False)
     sim.close()
     return {"workload": "elastic_cube", "particles": n, "grid": cfg.grid_size, "substeps": 1000,
-            "value": n * 1000 / el, "unit": "particle-substeps/s", "ms_per_step": 1e3 * el / 1000}
+            "value": n * 1000 / el, "unit": "particle-substeps/s", "ms_per_step": 1e3 * el / 1000,
+            "autotuned": tuned}
 
 
 def timed_diff(sim, dt, steps, every, loss_grad):
