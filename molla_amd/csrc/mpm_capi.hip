@@ -448,19 +448,23 @@ int sort_particles(mpm_sim* s, bool reorder) {
     HIP_TRY(s, mpm::radix_sort_pairs(s->keys, s->vals, s->n, bits, s->sort_scratch, s->stream));
     if (reorder) {
         const int other = 1 - s->ps_cur;
-        const uint32_t* src = s->vals;  // sorted previous slots
-        if (!by_slot) {
-            HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
-            src = s->src_slot;
-        }
-        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, src, s->n, s->stream));
-        HIP_TRY(s, mpm::launch_index_slots(s->ps[other], s->cap, s->slot_of_id, s->n, s->stream));
+        // The active list and the chunk list depend on the sorted keys only.
         HIP_TRY(s, mpm::launch_static_rebuild(s->d_scenes, s->tile_scene, s->tile_flag, s->slist,
                                               s->status + mpm::ST_SCOUNT, s->keys, s->n, s->tile_lb,
                                               s->total_tiles, s->smark, s->spos, s->sscan, s->stream));
         HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
                                             s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
                                             s->chunk_of_slot, s->stream));
+        // previous slot of every new slot: the sorted slots spread over cells
+        // inside each chunk, or the reference order's ids mapped to slots
+        if (by_slot)
+            HIP_TRY(s, mpm::launch_chunk_spread(s->d_scenes, s->schunks, s->status + mpm::ST_NCHUNKS,
+                                                s->max_schunks, s->ps[s->ps_cur], s->cap, s->vals, s->src_slot,
+                                                s->keys, s->stream));
+        else
+            HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
+        HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->src_slot, s->n, s->stream));
+        HIP_TRY(s, mpm::launch_index_slots(s->ps[other], s->cap, s->slot_of_id, s->n, s->stream));
         s->ps_cur = other;
         s->sorted = true;
         s->since_sort = 0;

@@ -175,6 +175,9 @@ hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_sc
                                  hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_out, int by_slot, hipStream_t s);
+hipError_t launch_chunk_spread(const SceneDev* scenes, const Chunk* chunks, const uint32_t* n_chunks,
+                               uint32_t max_chunks, const float* ps, uint32_t cap, const uint32_t* src_in,
+                               uint32_t* src_out, uint32_t* keys, hipStream_t s);
 hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
                           uint32_t n, hipStream_t s);
 hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,

@@ -903,6 +903,122 @@ hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint
     return hipGetLastError();
 }
 
+// Spreads the particles of each chunk of the sorted layout over cells: the
+// chunk's particles are ranked among those with the same stencil base cell
+// (ties by slot) and laid out rank-major, base cells ascending within a rank.
+// The sort alone leaves a chunk's particles in arbitrary order inside their
+// tiles, so the 64 lanes of a wave fall on ~125 base cells at random and the
+// scatter's LDS integer adds to one word from several lanes serialise; in
+// rank-major order a wave's lanes mostly hold distinct, neighbouring bases.
+// Only the order inside a chunk changes: the chunk list and its partial
+// boxes stay valid, and the fixed-point scatter sums do not depend on order.
+// src_in/src_out: previous slot of each sorted slot, before and after; the
+// sorted tile keys are permuted with them (in place), so each slot keeps the
+// key of its own particle for the scatter's escape test.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int mask) {
+    const uint32_t lo = __shfl_xor(uint32_t(x), mask, 64);
+    const uint32_t hi = __shfl_xor(uint32_t(x >> 32), mask, 64);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// Ascending bitonic sort of one value per thread of a 256-thread block:
+// returns the value at sorted position t.  Partners closer than a wave are
+// exchanged by shuffles; the three steps across waves go through v (LDS).
+// Every thread calls it; v may be read by other threads until the first
+// barrier.
+__device__ __forceinline__ uint64_t block_sort256(uint64_t x, uint64_t* v, uint32_t t) {
+    for (uint32_t k = 2; k <= CHUNK_PARTICLES; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            uint64_t other;
+            if (j >= 64u) {
+                __syncthreads();
+                v[t] = x;
+                __syncthreads();
+                other = v[t ^ j];
+            } else {
+                other = shfl_xor_u64(x, int(j));
+            }
+            // the lower index of a pair keeps the minimum in an ascending run
+            const bool keep_min = (t < (t ^ j)) == ((t & k) == 0u);
+            x = keep_min ? (x < other ? x : other) : (x < other ? other : x);
+        }
+    __syncthreads();  // the last cross-wave reads of v are done
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_chunk_spread(const SceneDev* __restrict__ scenes,
+                                                      const Chunk* __restrict__ chunks,
+                                                      const uint32_t* __restrict__ n_chunks,
+                                                      const float* __restrict__ ps, uint32_t cap,
+                                                      const uint32_t* __restrict__ src_in,
+                                                      uint32_t* __restrict__ src_out,
+                                                      uint32_t* __restrict__ keys) {
+    __shared__ uint64_t order[CHUNK_PARTICLES];
+    __shared__ uint32_t rank[CHUNK_PARTICLES];
+    __shared__ uint32_t tiles[CHUNK_PARTICLES];
+    const uint32_t nch = *n_chunks;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
+        const Chunk ch = chunks[ci];
+        const uint32_t cnt = min(ch.count, CHUNK_PARTICLES);
+        const SceneDev& sc = scenes[ch.scene];
+        uint32_t src = 0u, tile = 0u;
+        uint64_t base = ~0ull;  // 30-bit base cell key; unused threads sort last
+        if (t < cnt) {
+            src = src_in[ch.start + t];
+            tile = keys[ch.start + t];
+            uint32_t k = 0u;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float xp = (ps[(size_t)(CX + d) * cap + src] - sc.origin[d]) * sc.inv_dx;
+                const int b = int(floorf(xp - 0.5f)) - sc.window[d] + 1;
+                k |= (uint32_t(b) & 1023u) << (10 * d);
+            }
+            base = uint64_t(k);
+        }
+        // 1. (base, slot) ascending: a particle's rank is its distance from
+        //    the first entry of its base's run
+        const uint64_t e = block_sort256(base == ~0ull ? ~0ull : (base << 8) | t, order, t);
+        order[t] = e;
+        __syncthreads();
+        if (e != ~0ull) {
+            const uint64_t first = e & ~0xffull;
+            uint32_t lo = 0u, hi = t;  // lower bound of the run in order[0, t]
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (order[mid] < first) lo = mid + 1u;
+                else hi = mid;
+            }
+            rank[uint32_t(e & 0xffu)] = t - lo;
+        }
+        __syncthreads();
+        // 2. (rank, base, slot) ascending: the new position of every slot
+        const uint64_t f = block_sort256(base == ~0ull ? ~0ull : (uint64_t(rank[t]) << 38) | (base << 8) | t,
+                                         order, t);
+        const uint32_t from = uint32_t(f & 0xffu);
+        // hand the slot's previous slot and tile key to its new position
+        // (rank is free: every thread read its own entry before the sort's barriers)
+        rank[t] = src;
+        tiles[t] = tile;
+        __syncthreads();
+        if (t < cnt) {
+            src_out[ch.start + t] = rank[from];
+            keys[ch.start + t] = tiles[from];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_chunk_spread(const SceneDev* scenes, const Chunk* chunks, const uint32_t* n_chunks,
+                               uint32_t max_chunks, const float* ps, uint32_t cap, const uint32_t* src_in,
+                               uint32_t* src_out, uint32_t* keys, hipStream_t s) {
+    if (max_chunks == 0) return hipSuccess;
+    const unsigned blocks = std::min<unsigned>(max_chunks, 8192u);
+    hipLaunchKernelGGL(k_chunk_spread, dim3(blocks), dim3(256), 0, s, scenes, chunks, n_chunks, ps, cap, src_in,
+                       src_out, keys);
+    return hipGetLastError();
+}
+
 __global__ void k_index_slots(const float* __restrict__ ps, uint32_t cap, uint32_t* __restrict__ slot_of_id,
                               uint32_t n) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
