@@ -1272,7 +1272,7 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     float best = 1e30f, ms = 0.f;
     const uint32_t pbs[] = {0u, 1024u, 2048u, 4096u, 8192u, 16384u};
     const uint32_t gbs[] = {512u, 1024u, 2048u, 4096u, 8192u};
-    const uint32_t ses[] = {8u, 16u, 32u, 64u};
+    const uint32_t ses[] = {8u, 16u, 32u, 64u, 128u};
     const uint32_t gb0 = best_gb, se0 = best_se ? best_se : 16u;
     for (uint32_t pb : pbs) {
         if (rc != MPM_OK) break;

@@ -887,19 +887,30 @@ hipError_t launch_reorder_index(const uint32_t* sorted_ids, const uint32_t* slot
     return hipGetLastError();
 }
 
+// One slot per thread through all planes: the index is read once and every
+// thread keeps a plane group's loads in flight together.
 __global__ __launch_bounds__(256) void k_reorder(const float* __restrict__ src, float* __restrict__ dst,
                                                  uint32_t cap, const uint32_t* __restrict__ src_slot,
                                                  uint32_t n) {
-    const size_t plane = (size_t)blockIdx.y * cap;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
-        dst[plane + j] = src[plane + src_slot[j]];
+    constexpr int GROUP = 10;
+    static_assert(NCOMP % GROUP == 0, "plane groups");
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const size_t from = src_slot[j];
+        for (int c0 = 0; c0 < NCOMP; c0 += GROUP) {
+            float v[GROUP];
+#pragma unroll
+            for (int c = 0; c < GROUP; ++c) v[c] = src[(size_t)(c0 + c) * cap + from];
+#pragma unroll
+            for (int c = 0; c < GROUP; ++c) dst[(size_t)(c0 + c) * cap + j] = v[c];
+        }
+    }
 }
 
 hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
                           uint32_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    unsigned blocks = std::min<unsigned>((n + 255) / 256, 2048u);
-    hipLaunchKernelGGL(k_reorder, dim3(blocks, NCOMP), dim3(256), 0, s, src, dst, cap, src_slot, n);
+    unsigned blocks = std::min<unsigned>((n + 255) / 256, 16384u);
+    hipLaunchKernelGGL(k_reorder, dim3(blocks), dim3(256), 0, s, src, dst, cap, src_slot, n);
     return hipGetLastError();
 }
 

@@ -188,6 +188,11 @@ def test_sort_interval_and_write_vc_do_not_change_results_beyond_rounding():
         _, r = run_gpu(cfg, x, None, meta["dt"], 40, **kw)
         for f in ("x", "v", "F"):
             assert rel_err(r[f], base[f]) < {"x": 1e-6, "v": 1e-3, "F": 1e-5}[f]
+        if kw.get("sort_every") == 1:
+            # The sort spreads each chunk's particles over cells and moves
+            # the sorted tile keys with them; a key left behind would send
+            # particles of a slow scene to the global-atomic fallback.
+            assert r["status"]["sorts"] >= 2 and r["status"]["escapes"] == 0, r["status"]
 
 
 def test_graph_replays_match_eager():
