@@ -70,15 +70,28 @@ __device__ __forceinline__ void latch(uint32_t* status, uint32_t bits) {
     atomicOr(&status[ST_FAULT], bits);
 }
 
+// Wave-wide reductions with DPP lane moves (VALU only; a shuffle would be a
+// ds_bpermute through the LDS pipe, which the scatter keeps busy).  Within
+// each row of 16 lanes: swap neighbours, swap pairs, half-row mirror, row
+// mirror; then rows 1 and 3 take rows 0 and 2 (row_bcast:15), rows 2 and 3
+// take lane 31 (row_bcast:31), and lane 63 holds the result, read back as a
+// wave-uniform value.  Lanes a step leaves out read `identity`, so each step
+// fuses into one DPP min/max.  Every lane of the wave must be active.
+template <typename Op>
+__device__ __forceinline__ int wave_reduce_dpp(int v, int identity, Op op) {
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x142, 0xA, 0xF, false));  // row_bcast:15 into rows 1, 3
+    v = op(v, __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xC, 0xF, false));  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
 __device__ __forceinline__ int wave_min(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+    return wave_reduce_dpp(v, 0x7fffffff, [](int a, int b) { return min(a, b); });
 }
 __device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+    return wave_reduce_dpp(v, int(0x80000000u), [](int a, int b) { return max(a, b); });
 }
 
 // Material row of a particle (mpm.wgsl:152-157): global table at scene base.
@@ -148,10 +161,10 @@ __device__ __forceinline__ int exp2_ceil(float x) {
     return m == 0.5f ? e - 1 : e;
 }
 
+// Maximum of non-negative floats (+inf included), whose bit patterns order
+// as integers; a NaN lane counts as 0, as fmaxf would drop it.
 __device__ __forceinline__ float wave_maxf(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+    return __int_as_float(wave_max(v == v ? __float_as_int(v) : 0));
 }
 
 // One particle's 27-node P2G straight to global f32 atomics.
