@@ -3,15 +3,18 @@
 //
 // Chunks (one block of the particle pass each) are cut from the sorted
 // particle array so that a chunk never leaves its *segment*: one tile row of
-// one scene, four consecutive tiles along x.  A chunk's particles then stay
-// inside a 4x1x1-tile region (plus motion since the sort), so its P2G box
-// always fits the block's LDS budget.
+// one scene, eight consecutive tiles along x.  A chunk's particles then stay
+// inside an 8x1x1-tile region (plus motion since the sort), so its P2G box
+// (at most 34 x 6 x 6 cells at the sort) fits the block's LDS budget.  A
+// segment's last chunk is partly empty; eight tiles rather than four halve
+// the idle lanes this costs on dense scenes.
 #include <algorithm>
 
 #include "mpm_internal.hpp"
 
 namespace mpm {
 
+constexpr uint32_t SEGMENT_TILES = 8;  // tiles along x per segment (a power of two)
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -120,9 +123,9 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 __device__ __forceinline__ void segment_of(const SceneDev& sc, uint32_t key, uint32_t& first, uint32_t& ntiles) {
     const uint32_t tl = key - sc.tile_base;
     const uint32_t tx = tl % sc.W;
-    const uint32_t g0 = tx & ~3u;
+    const uint32_t g0 = tx & ~(SEGMENT_TILES - 1u);
     first = key - (tx - g0);
-    ntiles = min(4u, sc.W - g0);
+    ntiles = min(SEGMENT_TILES, sc.W - g0);
 }
 
 __global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
