@@ -90,7 +90,7 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 // A particle whose stencil has left that neighbourhood since the sort is
 // scattered with global atomics instead (and counted in ST_ESCAPES).
 // 1600 cells of four int64 channels (51 KB) lets three blocks share a CU's
-// 160 KB LDS; a chunk's box (eight tiles of one tile row plus the stencil
+// 160 KB LDS; a chunk's box (4 or 8 tiles of one tile row plus the stencil
 // reach and the motion since the last sort) stays well inside it, and a
 // larger box falls back to global atomics (counted in ST_FALLBACK).
 constexpr int LDS_CELLS = 1600;

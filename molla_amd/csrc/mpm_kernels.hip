@@ -10,7 +10,7 @@
 // The P2G half reduces the block's scatter in LDS (fixed point: 32-bit for the
 // shader model, 64-bit for the differentiable model) over
 // the bounding box of its particles' stencils -- particles are kept sorted by
-// 4^3 tile and cut into chunks that stay inside eight tiles of one tile row,
+// 4^3 tile and cut into chunks that stay inside 4 or 8 tiles of one tile row,
 // so that box is small -- and stores the box as the chunk's partial with
 // plain stores.  The next grid update sums the partials of the chunks around
 // each active tile in ascending chunk order.  Particles that moved beyond the

@@ -2,19 +2,21 @@
 // chunk list of the fused particle pass.
 //
 // Chunks (one block of the particle pass each) are cut from the sorted
-// particle array so that a chunk never leaves its *segment*: one tile row of
-// one scene, eight consecutive tiles along x.  A chunk's particles then stay
-// inside an 8x1x1-tile region (plus motion since the sort), so its P2G box
-// (at most 34 x 6 x 6 cells at the sort) fits the block's LDS budget.  A
-// segment's last chunk is partly empty; eight tiles rather than four halve
-// the idle lanes this costs on dense scenes.
+// particle array so that a chunk never leaves its *segment*: consecutive
+// tiles along x in one tile row of one scene.  A group of eight tiles that
+// holds at least DENSE_GROUP particles is one segment; a sparser group is two
+// segments of four.  A segment's last chunk is partly empty, and eight-tile
+// segments halve the idle lanes this costs on dense scenes, where a chunk
+// covers about one tile anyway.  In a sparse row a chunk may spread over its
+// whole segment, and four tiles keep its P2G box (18 x 6 x 6 cells at the
+// sort, plus the motion since) well inside the block's LDS budget.
 #include <algorithm>
 
 #include "mpm_internal.hpp"
 
 namespace mpm {
 
-constexpr uint32_t SEGMENT_TILES = 8;  // tiles along x per segment (a power of two)
+constexpr uint32_t DENSE_GROUP = 2 * CHUNK_PARTICLES;  // particles that make 8 tiles one segment
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -120,12 +122,22 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 
 // ---------------------------------------------------------------------------
 // Chunk list of the sorted particle array.
-__device__ __forceinline__ void segment_of(const SceneDev& sc, uint32_t key, uint32_t& first, uint32_t& ntiles) {
+// The segment of tile `key`: its first tile and its length along x.  lb is
+// the tile lower-bound table of the sorted keys.
+__device__ __forceinline__ void segment_of(const SceneDev& sc, const uint32_t* lb, uint32_t key, uint32_t& first,
+                                           uint32_t& ntiles) {
     const uint32_t tl = key - sc.tile_base;
     const uint32_t tx = tl % sc.W;
-    const uint32_t g0 = tx & ~(SEGMENT_TILES - 1u);
-    first = key - (tx - g0);
-    ntiles = min(SEGMENT_TILES, sc.W - g0);
+    const uint32_t g8 = tx & ~7u, n8 = min(8u, sc.W - g8);
+    const uint32_t first8 = key - (tx - g8);
+    if (lb[first8 + n8] - lb[first8] >= DENSE_GROUP) {
+        first = first8;
+        ntiles = n8;
+        return;
+    }
+    const uint32_t g4 = tx & ~3u;
+    first = key - (tx - g4);
+    ntiles = min(4u, sc.W - g4);
 }
 
 __global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
@@ -135,7 +147,7 @@ __global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_
         const uint32_t key = keys[j];
         const SceneDev& sc = scenes[tile_scene[key]];
         uint32_t first, nt;
-        segment_of(sc, key, first, nt);
+        segment_of(sc, lb, key, first, nt);
         const uint32_t seg_begin = lb[first];
         flags[j] = ((j - seg_begin) % CHUNK_PARTICLES) == 0u ? 1u : 0u;
     }
@@ -153,7 +165,7 @@ __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t
         const uint32_t scn = tile_scene[key];
         const SceneDev& sc = scenes[scn];
         uint32_t first, nt;
-        segment_of(sc, key, first, nt);
+        segment_of(sc, lb, key, first, nt);
         const uint32_t seg_end = lb[first + nt];
         Chunk ch;
         ch.scene = scn;
