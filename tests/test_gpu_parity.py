@@ -290,6 +290,32 @@ def test_cpu_diff_floor_row_on_a_rounding_tie():
     assert ex < 1e-5 and ev < 1e-3
 
 
+def test_sparse_scene_with_more_chunks_than_partial_slots_matches_oracle():
+    # A few thousand particles scattered over a 128^3 grid: most segments hold
+    # a particle or two, so the chunk list outgrows the partial buffer
+    # (cap/256 + cap/64 + scenes + 64 slots) and the chunks beyond it scatter
+    # with global atomics (status.fallback_chunks).  Results must still match
+    # the oracle.
+    rng = np.random.default_rng(61)
+    n = 3000
+    x = rng.uniform(0.1, 0.9, (n, 3)).astype(np.float32)
+    cfg = MpmConfig.sand().copy(grid_size=128, cell_size=1.0 / 128, grid_origin=(0.0, 0.0, 0.0),
+                                rheology=Viscous(0, 0), youngs=1.0e3, max_particles=n)
+    dt, steps = 1e-4, 20
+    _, g = run_gpu(cfg, x, None, dt, steps)
+    st = g["status"]
+    log(f"sparse scene: {st}")
+    assert st["fallback_chunks"] > 0 and st["fault_bits"] == 0
+    o64 = oracle_state(cfg, x)
+    o32 = oracle_state(cfg, x, dtype=np.float32)
+    orc.step(cfg, o64, dt, steps)
+    orc.step(cfg, o32, dt, steps)
+    for f, floor in (("x", 1e-6), ("v", 1e-3), ("F", 1e-5)):
+        e, e32 = rel_err(g[f], getattr(o64, f)), rel_err(getattr(o32, f), getattr(o64, f))
+        log(f"  {f}: gpu {e:.2e} f32-oracle {e32:.2e}")
+        assert e <= max(8 * e32, floor), (f, e, e32)
+
+
 def test_escaped_particles_fall_back_to_global_atomics():
     # Fast particles and a long sort interval: stencils leave their sorted
     # 3x3x3 tile neighbourhood, so those particle-substeps take the global
