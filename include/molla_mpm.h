@@ -103,7 +103,8 @@ typedef struct {
     uint32_t sorts;             /* particle re-sorts performed                */
     uint32_t escapes;           /* particle-substeps scattered by global atomics because the
                                    particle left its sorted tile neighbourhood */
-    uint32_t fallback_chunks;   /* block passes whose P2G box exceeded the LDS budget */
+    uint32_t fallback_chunks;   /* block passes that scattered with global atomics: P2G box
+                                   above the LDS budget, or chunk beyond the partial buffer */
     uint32_t chunks;            /* blocks of the sorted layout's fused pass   */
 } mpm_status_info;
 
