@@ -117,7 +117,8 @@ struct Scatter {
 //   narrow (int32): each contribution is rounded to 2^-23 of the power of
 //     two above the block's bound on its values -- about the precision of f32
 //     accumulation for the forward P2G, whose contributions in a block are of
-//     one magnitude;
+//     one magnitude.  Two channels share one 64-bit word per cell, so one
+//     ds_add_u64 adds both (see pack_lo_hi);
 //   wide (int64): each f32 contribution converts exactly (2^-50 of the
 //     largest), for sums whose terms span many magnitudes and cancel (the
 //     adjoint's dJ/d grid v).
@@ -132,6 +133,19 @@ struct ScatterLdsT {
     double scale[2];  // power-of-two fixed-point scale of the mass and momentum channels
 };
 using ScatterLds = ScatterLdsT<false>;
+
+// Two int32 contributions as one 64-bit addend, lo + hi * 2^32 (mod 2^64).
+// Sums of such words give sum(lo) + sum(hi) * 2^32, from which both sums are
+// recovered exactly when each fits an int32 (unpack_lo_hi): the low half is
+// sum(lo) mod 2^32, and the rest is sum(hi).  The narrow scatter's sums of
+// 256 contributions below 2^22 each stay below 2^30.
+__device__ __forceinline__ unsigned long long pack_lo_hi(int lo, int hi) {
+    return (unsigned long long)(long long)lo + ((unsigned long long)(uint32_t)hi << 32);
+}
+__device__ __forceinline__ void unpack_lo_hi(unsigned long long s, int& lo, int& hi) {
+    lo = (int)(uint32_t)s;
+    hi = (int)(uint32_t)((s - (unsigned long long)(long long)lo) >> 32);
+}
 
 // round(a * b) as an int32 for |a * b| < 2^22: one f32 FMA with 1.5 * 2^23
 // rounds the product once and leaves the integer in the low mantissa bits.
@@ -285,11 +299,19 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         // narrow: node values are pre-scaled and rounded by fixed32; wide:
         // node values stay unscaled and fixed64 scales them exactly
         const float sm = WIDE ? 1.0f : (float)sm64, sp = WIDE ? 1.0f : (float)sp64;
+        // narrow: the four int32 rows hold two 64-bit words per cell,
+        // (m, px) in pairs[i] and (py, pz) in pairs[LDS_CELLS + i]
+        unsigned long long* pairs = reinterpret_cast<unsigned long long*>(&L.acc[0][0]);
         for (int i = tid; i < vol; i += BLOCK) {
-            if (MASS) L.acc[0][i] = 0;
-            L.acc[1][i] = 0;
-            L.acc[2][i] = 0;
-            L.acc[3][i] = 0;
+            if constexpr (WIDE) {
+                if (MASS) L.acc[0][i] = 0;
+                L.acc[1][i] = 0;
+                L.acc[2][i] = 0;
+                L.acc[3][i] = 0;
+            } else {
+                pairs[i] = 0ull;
+                pairs[LDS_CELLS + i] = 0ull;
+            }
         }
         __syncthreads();
         if (lds_on) {
@@ -339,10 +361,8 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                                     lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
                                     lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
                                 } else {
-                                    if (MASS) lds_add_i32(&L.acc[0][lrow + i], fixed32(wt, ms));
-                                    lds_add_i32(&L.acc[1][lrow + i], fixed32(wt, q0));
-                                    lds_add_i32(&L.acc[2][lrow + i], fixed32(wt, q1));
-                                    lds_add_i32(&L.acc[3][lrow + i], fixed32(wt, q2));
+                                    lds_add_u64(&pairs[lrow + i], pack_lo_hi(MASS ? fixed32(wt, ms) : 0, fixed32(wt, q0)));
+                                    lds_add_u64(&pairs[LDS_CELLS + lrow + i], pack_lo_hi(fixed32(wt, q1), fixed32(wt, q2)));
                                 }
                             }
                             q0 += a0[0];
@@ -370,9 +390,12 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         } else {
             // int32 sums: one rounding to f32, then an exact power-of-two scale
             const float im = (float)(1.0 / sm64), ip = (float)(1.0 / sp64);
-            for (int i = tid; i < vol; i += BLOCK)
-                part[i] = make_float4(MASS ? (float)(int)L.acc[0][i] * im : 0.f, (float)(int)L.acc[1][i] * ip,
-                                      (float)(int)L.acc[2][i] * ip, (float)(int)L.acc[3][i] * ip);
+            for (int i = tid; i < vol; i += BLOCK) {
+                int sm, sx, sy, sz;
+                unpack_lo_hi(pairs[i], sm, sx);
+                unpack_lo_hi(pairs[LDS_CELLS + i], sy, sz);
+                part[i] = make_float4(MASS ? (float)sm * im : 0.f, (float)sx * ip, (float)sy * ip, (float)sz * ip);
+            }
         }
         __syncthreads();
     }
