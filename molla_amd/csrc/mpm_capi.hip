@@ -27,12 +27,15 @@ namespace {
 
 constexpr uint32_t CHUNK = 256;
 
+// Everything a captured step_n bakes in: the launches' arguments and the
+// positions of its sorts (which follow from the interval and the substeps
+// since the last sort).
 struct GraphKey {
-    uint32_t dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks;
+    uint32_t dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks, sort_every, has_E;
     bool operator<(const GraphKey& o) const {
-        return std::tie(dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks) <
-               std::tie(o.dt_bits, o.n, o.mode, o.ps_cur, o.list_cur, o.dirty, o.sort_start, o.write_vc,
-                        o.pblocks, o.gblocks);
+        return std::tie(dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks, sort_every,
+                        has_E) < std::tie(o.dt_bits, o.n, o.mode, o.ps_cur, o.list_cur, o.dirty, o.sort_start,
+                                          o.write_vc, o.pblocks, o.gblocks, o.sort_every, o.has_E);
     }
 };
 
@@ -94,6 +97,11 @@ struct mpm_sim {
     bool boxes_valid = false;  // chunk_box holds the P2G boxes of the current positions
     uint32_t since_sort = 0;
     uint32_t sorts = 0;
+    // Bumped by everything that rewrites the sort state (keys, slot order,
+    // chunk list, tile table) or the planes wholesale: a sort of any kind and
+    // the autotuner's trials.  Unlike `sorts`, nothing ever puts it back, so
+    // the adjoint can tell that its forward's layout is gone.
+    uint64_t layout_gen = 0;
     int has_E = 0;
     // options
     int p2g_mode = MPM_P2G_FUSED;
@@ -120,6 +128,7 @@ struct mpm_sim {
     float diff_dt = 0.f;
     uint32_t diff_sorts = 0;
     int diff_ps = 0;
+    uint64_t diff_gen = 0;     // layout_gen when the forward finished
     float* tape = nullptr;     // pre-states of one checkpoint segment (NCOMP planes each)
     float4* gtape = nullptr;   // per substep of the segment: grid v, then P2G total (m, p)
     float* atape = nullptr;    // per substep of the segment: the P2G affine A (9 planes)
@@ -425,6 +434,7 @@ __global__ void k_iota(uint32_t* p, uint32_t n) {
 // keys[id] -> stable sort -> reorder particle planes into tile order.
 int sort_particles(mpm_sim* s, bool reorder) {
     s->boxes_valid = false;  // the chunk list is rebuilt
+    s->layout_gen++;
     if (s->dry) {
         if (reorder) {
             s->ps_cur = 1 - s->ps_cur;
@@ -932,6 +942,7 @@ int diff_forward(mpm_sim* s, float dt, uint32_t steps, uint32_t every) {
     s->diff_dt = dt;
     s->diff_sorts = s->sorts;
     s->diff_ps = s->ps_cur;
+    s->diff_gen = s->layout_gen;
     return MPM_OK;
 }
 
@@ -1135,8 +1146,10 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
 
 int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     if (s->diff_steps == 0) return fail(s, MPM_ERR_INVALID, "mpm_diff_forward must run first");
-    if (s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
-        return fail(s, MPM_ERR_INVALID, "the particle order changed since mpm_diff_forward");
+    if (s->layout_gen != s->diff_gen || s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
+        return fail(s, MPM_ERR_INVALID,
+                    "the particle order or sort state changed since mpm_diff_forward (a step, sort, key query "
+                    "or autotune ran in between)");
     if (!gx) return fail(s, MPM_ERR_INVALID, "grad_x_final must not be NULL");
     const uint32_t every = s->diff_every, steps = s->diff_steps;
     const uint32_t nseg = (steps + every - 1) / every;
@@ -1239,6 +1252,10 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     HIP_TRY(s, hipMemcpyAsync(st, s->status, sizeof(st), hipMemcpyDeviceToHost, s->stream));
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     const uint32_t sorts0 = s->sorts;
+    // host flags the trials' steps set that belong to the caller's state: a
+    // stepped handle reads position/velocity w as 0, and a reset of a latched
+    // fault blocks an archive until the caller steps or restores
+    const bool w_stepped0 = s->w_stepped, fault_reset0 = s->fault_reset;
     const int prof = s->profiling;
     s->profiling = 0;
     hipEvent_t ea = nullptr, eb = nullptr;
@@ -1307,6 +1324,11 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     s->gblocks = best_gb;
     s->sort_every = best_se;
     s->sorts = sorts0;
+    s->w_stepped = w_stepped0;
+    s->fault_reset = fault_reset0;
+    // the trials rebuilt keys, chunks and the slot order: a forward taped
+    // before this call cannot be reversed (layout_gen stays advanced)
+    s->layout_gen++;
     // counters of the trials are not the caller's: put back the fault latch,
     // gathers, escapes and fallback counters
     const int keep[] = {mpm::ST_FAULT, mpm::ST_GATHERS, mpm::ST_ESCAPES, mpm::ST_FALLBACK};
@@ -1500,7 +1522,7 @@ int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
                             (!sim->sorted || sim->sort_every == 0 || sim->since_sort >= sim->sort_every);
     GraphKey key{dtb, substeps, uint32_t(sim->p2g_mode), uint32_t(sim->ps_cur), uint32_t(sim->list_cur),
                  uint32_t(sim->grid_dirty), uint32_t(sort_start) | (sim->since_sort << 1), uint32_t(sim->write_vc),
-                 sim->pblocks, sim->gblocks};
+                 sim->pblocks, sim->gblocks, sim->sort_every, uint32_t(sim->has_E)};
     auto it = sim->graphs.find(key);
     if (it == sim->graphs.end()) {
         // Lazily allocated tables must exist before capture.

@@ -269,15 +269,15 @@ def step(cfg, st: State, dt: float, substeps: int, want_grid=False):
 
 
 def step_threads(cfg, st: State, dt: float, substeps: int) -> int:
-    """The SHADER step in f32 on all host threads (OpenMP), in place: a timing
-    port for the CPU baseline, whose sums run in another order than ``step``'s.
-    Returns the thread count used."""
-    if st.dtype != np.float32:
-        raise ValueError("step_threads runs in f32")
+    """The SHADER step on all host threads (OpenMP), in place, in the state's
+    precision: in f32 a timing port for the CPU baseline, in f64 a fast FP64
+    reference for long runs of large scenes.  Its sums run in another order
+    than ``step``'s.  Returns the thread count used."""
     s = scene_from(cfg)
-    ct = C.c_float
+    ct = C.c_float if st.dtype == np.float32 else C.c_double
+    fn = lib().orc_step_threads_f32 if st.dtype == np.float32 else lib().orc_step_threads_f64
     threads = C.c_uint32(0)
-    rc = lib().orc_step_threads_f32(C.byref(s), C.c_uint32(st.n), _p(st.x, ct), _p(st.v, ct), _p(st.C, ct),
+    rc = fn(C.byref(s), C.c_uint32(st.n), _p(st.x, ct), _p(st.v, ct), _p(st.C, ct),
                                     _p(st.F, ct), _p(st.jp, ct), _p(st.mass, ct), _p(st.vol, ct),
                                     _p(st.mat, C.c_uint32), _p(st.E, ct), ct(dt), C.c_uint32(substeps),
                                     _p(st.status, C.c_uint32), C.byref(threads))
@@ -322,6 +322,19 @@ def rigid_grid(cfg, dt: float, rows, grid_m, grid_p):
     lib().orc_rigid_grid_f64(C.byref(s), C.c_double(dt), _p(rows, C.c_float), C.c_uint32(nb), _p(m, C.c_double),
                              _p(p, C.c_double), _p(v, C.c_double), _p(mask, C.c_uint8), _p(wrench, C.c_double))
     return m, p, v, mask, wrench[:nb]
+
+
+def grid_update(cfg, dt: float, grid_m, grid_p):
+    """The shader grid update alone (mass gate, gravity, the six padded walls,
+    and the implicit sweeps when the config asks for them) on a dense grid
+    (m: G^3, p: G^3 x 3, cell = x + G (y + G z)); returns v (G^3 x 3)."""
+    return rigid_grid(cfg, dt, np.zeros((0, 20), np.float32), grid_m, grid_p)[2]
+
+
+def set_mutation(m: int) -> None:
+    """Plant deliberate mistake `m` in the oracle (0 = none): for tests that
+    show a closed-form check fails when the rule it pins is perturbed."""
+    lib().orc_set_mutation(C.c_int(int(m)))
 
 
 def implicit_sweeps(cfg, dt: float, grid_m, grid_v):

@@ -36,6 +36,22 @@
 
 #include <omp.h>
 
+// Planted mistakes (orc_set_mutation), for the tests that show a closed-form
+// check fails when the rule it pins is perturbed.  0 = the faithful oracle.
+enum Mutation : int {
+    MUT_NONE = 0,
+    MUT_PAD_ONE = 1,          // wall pad of 1 cell instead of 2
+    MUT_FRICTION_NORMAL = 2,  // boundary friction also scales the normal component
+    MUT_FRICTION_ONE_TAN = 3, // boundary friction on one tangential component only
+    MUT_CLAMP_LO = 4,         // position clamp low bound at window + 1
+    MUT_CLAMP_KEEP_V = 5,     // position clamp keeps the offending velocity
+    MUT_STRESS_CLAMP = 6,     // stress sigma clamp at [0.05, 10]
+    MUT_PLASTIC_CLAMP = 7,    // plasticity sigma clamp at [0.05, 10]
+    MUT_FLUID_SQRT = 8,       // fluid reset to sqrt(J) I
+    MUT_STRESS_COF = 9,       // volumetric stress term on R instead of the cofactor
+};
+static int g_mut = MUT_NONE;
+
 namespace {
 
 template <typename T>
@@ -350,12 +366,13 @@ M3<T> shader_stress(const M3<T>& F, T mu_h, T lam_h, T dt, T vol) {
     M3<T> U, V;
     V3<T> s;
     svd3_shader(F, U, s, V);
-    for (int i = 0; i < 3; ++i) s[i] = std::min(std::max(s[i], T(0.1)), T(10));
+    const T slo = g_mut == MUT_STRESS_CLAMP ? T(0.05) : T(0.1);
+    for (int i = 0; i < 3; ++i) s[i] = std::min(std::max(s[i], slo), T(10));
     M3<T> R = mul(U, tr(V));
     T J = s[0] * s[1] * s[2];
     M3<T> Fc = udv(U, s, V);
     V3<T> cd = {{s[1] * s[2], s[0] * s[2], s[0] * s[1]}};
-    M3<T> cof = udv(U, cd, V);
+    M3<T> cof = g_mut == MUT_STRESS_COF ? R : udv(U, cd, V);
     M3<T> pk1 = add(scale(T(2) * mu_h, sub(Fc, R)), scale(lam_h * (J - T(1)), cof));
     return scale(-dt * vol, mul(pk1, tr(Fc)));
 }
@@ -468,7 +485,8 @@ bool plastic(const orc_scene* sc, const Mat<T>& m, T youngs, const V3<T>& s, T j
         }
     }
     if (!finite3(ns.x, 3) || !std::isfinite(dq) || !std::isfinite(jp_in + dq)) return false;
-    for (int i = 0; i < 3; ++i) s_out[i] = std::min(std::max(ns[i], T(0.05)), T(20));
+    const T shi = g_mut == MUT_PLASTIC_CLAMP ? T(10) : T(20);
+    for (int i = 0; i < 3; ++i) s_out[i] = std::min(std::max(ns[i], T(0.05)), shi);
     jp_out = jp_replace ? jp_new : std::min(std::max(jp_in + dq, T(0)), T(5));
     return true;
 }
@@ -611,13 +629,21 @@ void grid_update_shader(const orc_scene* s, T dt, Grid<T>& g, bool threads = fal
                     continue;
                 }
                 v[1] = v[1] + dt * T(s->gravity_y);
-                const int pad = 2;
-                if (ix < pad) { v[0] = std::max(v[0], T(0)); v[1] *= bf; v[2] *= bf; }
-                if (ix >= n - pad) { v[0] = std::min(v[0], T(0)); v[1] *= bf; v[2] *= bf; }
-                if (iy < pad) { v[1] = std::max(v[1], T(0)); v[0] *= bf; v[2] *= bf; }
-                if (iy >= n - pad) { v[1] = std::min(v[1], T(0)); v[0] *= bf; v[2] *= bf; }
-                if (iz < pad) { v[2] = std::max(v[2], T(0)); v[0] *= bf; v[1] *= bf; }
-                if (iz >= n - pad) { v[2] = std::min(v[2], T(0)); v[0] *= bf; v[1] *= bf; }
+                const int pad = g_mut == MUT_PAD_ONE ? 1 : 2;
+                // a wall: the normal component may only point inwards, both
+                // tangential components are scaled by the friction
+                auto wall = [&](int a, bool low) {
+                    v[a] = low ? std::max(v[a], T(0)) : std::min(v[a], T(0));
+                    if (g_mut == MUT_FRICTION_NORMAL) v[a] *= bf;
+                    v[(a + 1) % 3] *= bf;
+                    if (g_mut != MUT_FRICTION_ONE_TAN) v[(a + 2) % 3] *= bf;
+                };
+                if (ix < pad) wall(0, true);
+                if (ix >= n - pad) wall(0, false);
+                if (iy < pad) wall(1, true);
+                if (iy >= n - pad) wall(1, false);
+                if (iz < pad) wall(2, true);
+                if (iz >= n - pad) wall(2, false);
                 for (int a = 0; a < 3; ++a) g.v[3 * c + a] = v[a];
             }
 }
@@ -723,9 +749,9 @@ bool g2p_shader(const orc_scene* s, Particles<T>& P, uint32_t p, T dt, const Gri
     for (int a = 0; a < 3; ++a) np[a] = x[a] + dt * vn[a];
     if (!finite3(np, 3) || !finite3(vn.x, 3) || !finite3(Cn.a, 9)) return false;
     for (int a = 0; a < 3; ++a) {
-        T lo = T(s->origin[a]) + T(s->window[a] + 2) * dx;
+        T lo = T(s->origin[a]) + T(s->window[a] + (g_mut == MUT_CLAMP_LO ? 1 : 2)) * dx;
         T hi = T(s->origin[a]) + T(s->window[a] + int(s->grid_size) - 2 - 1) * dx;
-        if (np[a] < lo || np[a] > hi) vn[a] = T(0);
+        if ((np[a] < lo || np[a] > hi) && g_mut != MUT_CLAMP_KEEP_V) vn[a] = T(0);
         np[a] = std::min(std::max(np[a], lo), hi);
     }
     M3<T> A = add(eye3<T>(), scale(dt, Cn));
@@ -751,7 +777,7 @@ bool plasticity_shader(const orc_scene* s, Particles<T>& P, uint32_t p) {
         double J = det3(M3<double>{{double(F.a[0]), double(F.a[1]), double(F.a[2]), double(F.a[3]),
                                     double(F.a[4]), double(F.a[5]), double(F.a[6]), double(F.a[7]),
                                     double(F.a[8])}});
-        double sc = std::cbrt(J);
+        double sc = g_mut == MUT_FLUID_SQRT ? std::sqrt(J) : std::cbrt(J);
         sc = std::min(std::max(sc, 0.05), 20.0);
         if (!std::isfinite(sc)) return false;
         M3<T> Fn = scale(T(sc), eye3<T>());
@@ -1503,6 +1529,18 @@ int orc_step_threads_f32(const orc_scene* s, uint32_t n, float* x, float* v, flo
     Particles<float> P{n, x, v, C, F, jp, mass, vol, E, mat};
     if (threads) *threads = uint32_t(omp_get_max_threads());
     return step_shader_threads<float>(s, P, dt, substeps, status);
+}
+// The same threaded step in f64: a fast FP64 reference for long runs of
+// large scenes (its sums differ from orc_step_f64's only in order).
+void orc_set_mutation(int m) { g_mut = m; }
+
+int orc_step_threads_f64(const orc_scene* s, uint32_t n, double* x, double* v, double* C, double* F, double* jp,
+                         const double* mass, const double* vol, const uint32_t* mat, const double* E, double dt,
+                         uint32_t substeps, uint32_t* status, uint32_t* threads) {
+    if (!s || s->grid_size == 0 || s->model == ORC_MODEL_CPUDIFF) return -2;
+    Particles<double> P{n, x, v, C, F, jp, mass, vol, E, mat};
+    if (threads) *threads = uint32_t(omp_get_max_threads());
+    return step_shader_threads<double>(s, P, dt, substeps, status);
 }
 
 // orc_step_* with rigid bodies (rows: 20 floats per body, see Rigids).

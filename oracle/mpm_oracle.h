@@ -115,6 +115,16 @@ int orc_step_threads_f32(const orc_scene* s, uint32_t n, float* x, float* v, flo
                          float* jp, const float* mass, const float* vol, const uint32_t* mat,
                          const float* E, float dt, uint32_t substeps, uint32_t* status,
                          uint32_t* threads);
+/* Plants one deliberate mistake in the rule it names (the Mutation enum in
+ * mpm_oracle.cpp; 0 = none), so a test can show that its closed-form check
+ * fails when that rule is perturbed.  Process-global; tests reset it to 0. */
+void orc_set_mutation(int m);
+/* The same threaded SHADER step in f64: a fast FP64 reference for long runs
+ * of large scenes (sums in another order than orc_step_f64's). */
+int orc_step_threads_f64(const orc_scene* s, uint32_t n, double* x, double* v, double* C, double* F,
+                         double* jp, const double* mass, const double* vol, const uint32_t* mat,
+                         const double* E, double dt, uint32_t substeps, uint32_t* status,
+                         uint32_t* threads);
 /* P2G only (shader or cpudiff model per s->model), scatter order = particle order. */
 int orc_p2g_f64(const orc_scene* s, uint32_t n, const double* x, const double* v, const double* C,
                 const double* F, const double* jp, const double* mass, const double* vol,

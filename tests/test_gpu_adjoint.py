@@ -67,6 +67,44 @@ def _oracle_run(cfg, d, dt, steps, gx, gv):
     return out
 
 
+def test_backward_refuses_once_the_forward_layout_is_gone():
+    # The tape of a forward is tied to the slot order, keys and chunk list of
+    # its sort.  The autotuner's trials and a key query rebuild them (the
+    # autotuner then puts the sort counter back), so a backward after either
+    # must refuse rather than reverse with the wrong layout; a new forward
+    # then reverses as if nothing had happened.
+    from molla_amd import MpmError
+
+    cfg, d, meta = _scene(2048)
+    d = _perturbed(d, 3)
+    dt, n = meta["dt"], len(d["x"])
+    gx = np.full((n, 3), 1.0 / n, np.float32)
+    ref = _gpu_run(cfg, d, dt, 8, 4, gx, None)
+    for disturb in ("autotune", "keys"):
+        log(f"forward, {disturb}, backward")
+        sim = MpmSimulation(cfg, max_particles=n)
+
+        def reset():
+            sim.set_particles(d["x"], d["v"])
+            sim.write_field("youngs", d["E"])
+            sim.write_field("F", d["F"].reshape(-1, 9))
+
+        reset()
+        sim.diff_forward(dt, 8, 4)
+        if disturb == "autotune":
+            sim.autotune(dt, 2)
+        else:
+            sim.sort_keys_debug()
+        with pytest.raises(MpmError):
+            sim.diff_backward(gx)
+        reset()
+        sim.diff_forward(dt, 8, 4)
+        g = sim.diff_backward(gx)
+        sim.close()
+        for f in ("x", "v", "E"):
+            assert rel_err(g[f], ref[f]) < 1e-6, (disturb, f, rel_err(g[f], ref[f]))
+
+
 def test_adjoint_matches_fp64_oracle():
     cfg, d, meta = _scene()
     d = _perturbed(d, 21)

@@ -210,6 +210,31 @@ def test_fault_cannot_be_archived_or_hidden_by_status_reset():
     sim.close()
 
 
+def test_autotune_keeps_the_archive_rules():
+    # The autotuner's trial substeps are not the caller's steps: a reset of a
+    # latched fault still blocks an archive afterwards, and a restored handle
+    # still captures its own position/velocity w words.
+    cfg = _state(1).cfg.copy(gravity_y=0.0)
+    sim = _sim(cfg)
+    x = np.array([[-0.69, 0.11, -0.69], [0.0, 0.5, 0.0]], np.float32)
+    v = np.array([[0, 0, 0], [np.nan, 0, 0]], np.float32)
+    sim.set_particles(x, v)
+    sim.step_n(1e-4, 1)
+    assert sim.read_status()["faulted"]
+    sim.reset_status()
+    sim.autotune(1e-4, 2)
+    assert not sim.read_status()["faulted"]
+    with pytest.raises(MpmError):
+        sim.checkpoint_soil()
+    sim.close()
+    st = _state(129)
+    sim = _sim(st.cfg)
+    sim.restore_soil(st)
+    sim.autotune(1e-4, 2)
+    assert sim.checkpoint_soil().to_bytes() == st.to_bytes()
+    sim.close()
+
+
 def test_large_state_restore_capture_is_idempotent():
     # 262,144 particles of mixed material: capture(restore(capture)) is exact
     # and a restored copy continues exactly like the source in deterministic mode

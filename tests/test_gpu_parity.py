@@ -212,6 +212,79 @@ def test_graph_replays_match_eager():
         _ = eager.read_field("F")  # eager work between replays
 
 
+def test_fused_graph_replays_with_sorts_inside_match_eager():
+    # The batch number is measured with step_n_graph in the fused mode, whose
+    # captured window bakes in where its sorts fall (mpm_command_batch.rs
+    # :157-216 is the reference's prepared-step contract).  With a sort every
+    # 3 substeps and 8 substeps per call, every replay holds two or three
+    # sorts at offsets that change from call to call; the eager handle steps
+    # the same calls.  Changing the sort interval and writing a per-particle
+    # E between calls must not replay a graph captured before.
+    cfg, x, meta = workloads.elastic_cube(4096, seed=4)
+    dt = meta["dt"]
+    eager = MpmSimulation(cfg, max_particles=len(x))
+    graph = MpmSimulation(cfg, max_particles=len(x))
+    for s in (eager, graph):
+        s.set_particles(x)
+        s.set_sort_interval(3)
+    E = np.full(len(x), 1.5 * cfg.youngs, np.float32)
+    for rep in range(9):
+        if rep == 4:
+            for s in (eager, graph):
+                s.set_sort_interval(2)
+        if rep == 6:
+            for s in (eager, graph):
+                s.write_field("youngs", E)
+        log(f"fused graph replay {rep}")
+        eager.step_n(dt, 8)
+        graph.step_n_graph(dt, 8)
+        for f in ("x", "v", "F"):
+            a = eager.read_field({"x": "position", "v": "velocity", "F": "F"}[f])
+            b = graph.read_field({"x": "position", "v": "velocity", "F": "F"}[f])
+            assert np.array_equal(a, b), (rep, f, float(np.abs(a - b).max()))
+        se, sg = eager.read_status(), graph.read_status()
+        assert se["escapes"] == 0 and sg["sorts"] == se["sorts"], (se, sg)
+    assert se["sorts"] >= 20
+    eager.close()
+    graph.close()
+
+
+def test_batched_graph_replays_match_eager_and_oracle():
+    # A sample of the 512-scene batch stepped by graph replays (fused mode,
+    # sorts inside the window) equals eager stepping bit for bit, and each
+    # scene matches its own FP64 oracle run.
+    cfgs, xs, meta = workloads.batched_scenes(8, seed=13, n_lo=1000, n_hi=6000)
+    dt, calls, k = meta["dt"], 4, 6
+    log(f"batch of {len(cfgs)} scenes, {meta['particles']} particles, grids {meta['grid_min']}-{meta['grid_max']}")
+    eager = MpmSimulation(cfgs)
+    graph = MpmSimulation(cfgs)
+    for s in (eager, graph):
+        s.set_particles_batch(xs)
+        s.set_sort_interval(4)
+    for rep in range(calls):
+        eager.step_n(dt, k)
+        graph.step_n_graph(dt, k)
+        a, b = eager.read_positions(), graph.read_positions()
+        assert np.array_equal(a, b), rep
+        assert np.array_equal(eager.read_field("F"), graph.read_field("F")), rep
+    pos = graph.read_positions()
+    st = graph.read_status()
+    assert st["sorts"] >= calls * k // 4 and not st["faulted"], st
+    off = 0
+    for c, x in zip(cfgs, xs):
+        o64 = oracle_state(c, x)
+        o32 = oracle_state(c, x, dtype=np.float32)
+        orc.step(c, o64, dt, calls * k)
+        orc.step(c, o32, dt, calls * k)
+        e = rel_err(pos[off:off + len(x)], o64.x)
+        e32 = rel_err(o32.x, o64.x)
+        log(f"scene of {len(x)}: x err {e:.2e} (f32 oracle {e32:.2e})")
+        assert e <= max(8 * e32, 1e-6)
+        off += len(x)
+    eager.close()
+    graph.close()
+
+
 def test_fault_latch_freezes_state():
     cfg = MpmConfig.sand()
     x = fill_box([-0.1, 0.3, -0.1], [0.1, 0.5, 0.1], 4, 2)
