@@ -141,6 +141,21 @@ def time_steps(sim, dt, k, graph):
     return time.perf_counter() - t0
 
 
+def time_sort(sim, dt, graph):
+    """Wall time of one layout sort: a substep forced to sort (interval 1,
+    right after a substep) minus a substep without one, best of three."""
+    best = []
+    for force in (True, False):
+        ts = []
+        for _ in range(3):
+            sim.set_sort_interval(1 if force else 1 << 20)
+            if force:
+                sim.step_n(dt, 1)  # since_sort = 1 >= 1: the next substep sorts
+            ts.append(time_steps(sim, dt, 1, False))
+        best.append(min(ts))
+    return max(best[0] - best[1], 0.0)
+
+
 def cpu_baseline(cfg, d, dt, substeps, gpu_state):
     """The CPU legs on one bounded sample (the first `substeps` substeps of the
     workload): the FP64 oracle on one core (faithful), the f32 threaded port
@@ -171,6 +186,88 @@ def cpu_baseline(cfg, d, dt, substeps, gpu_state):
             "fair_seconds": el_fair, "threads": threads, "gpu_error": err, "fair_error": err32}
 
 
+def cpu_legs(scale, out):
+    """CPU timings of the other workloads on this host, beside the GPU's:
+    the elastic cube in full (1,000 substeps) with the FP64 oracle on one core
+    and the f32 threaded port on every host thread; the differentiable scene
+    and a sample of the differentiable batch through the oracle's CPUDIFF
+    forward and reverse (mpm_cpu_diff.rs step_once_record and
+    mpm_adjoint.rs backward restated; single thread, f64, as the reference)."""
+    import oracle as orc
+    from molla_amd import workloads
+
+    legs = {}
+    # elastic cube, whole workload
+    n = max(1024, int(8192 * min(scale, 1.0)))
+    cfg, x, meta = workloads.elastic_cube(n, seed=1)
+    steps = meta["substeps"]
+    gpu = out.get("plain", {}).get("value")
+    for dtype, name in ((np.float64, "elastic_cube_fp64"), (np.float32, "elastic_cube_f32_threads")):
+        st = orc.State(x, mass=np.full(n, cfg.particle_mass), vol=np.full(n, cfg.particle_volume), dtype=dtype)
+        t0 = time.perf_counter()
+        if dtype == np.float64:
+            orc.step(cfg, st, meta["dt"], steps)
+            threads = 1
+        else:
+            threads = orc.step_threads(cfg, st, meta["dt"], steps)
+        el = time.perf_counter() - t0
+        v = n * steps / el
+        legs[name] = {"value": v, "unit": "particle-substeps/s", "cores": threads,
+                      "kind": "port", "sample": f"whole workload: {n} particles x {steps} substeps, {el:.1f} s",
+                      "gpu_over_cpu": gpu / v if gpu else None}
+        log(f"cpu leg {name}: {legs[name]}")
+
+    def diff_leg(cfgs, states, dt, steps, grads):
+        tf = tb = 0.0
+        for c, st, g in zip(cfgs, states, grads):
+            st0 = st.copy()
+            t0 = time.perf_counter()
+            tape = orc.diff_forward(c, st, dt, steps)
+            tf += time.perf_counter() - t0
+            gx = g(st)
+            t0 = time.perf_counter()
+            orc.diff_backward(c, st0, dt, steps, tape, gx)
+            tb += time.perf_counter() - t0
+        return tf, tb
+
+    # differentiable scene: the first 16 of its 64 substeps (one checkpoint
+    # segment), forward with tape and reverse
+    n = max(4096, int(65536 * scale))
+    cfg, d, meta = workloads.diff_scene(n)
+    k = 16
+    target = np.asarray(meta["target"])
+    st = orc.State(d["x"], d["v"], mass=np.full(n, cfg.particle_mass), vol=np.full(n, cfg.particle_volume),
+                   E=d["E"])
+    tf, tb = diff_leg([cfg], [st], meta["dt"], k,
+                      [lambda s: np.tile((2.0 / n) * (s.x.mean(axis=0) - target), (n, 1))])
+    ga = out.get("adjoint", {})
+    legs["diff_scene"] = {"forward_value": n * k / tf, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+                          "adjoint_fwd_ratio": tb / tf,
+                          "sample": f"first {k} of {meta['substeps']} substeps, {n} particles: forward {tf:.2f} s, "
+                                    f"reverse {tb:.2f} s",
+                          "gpu_over_cpu_forward": (n * meta["substeps"] / (1e-3 * ga["forward_ms"])) / (n * k / tf)
+                          if "forward_ms" in ga else None}
+    log(f"cpu leg diff_scene: {legs['diff_scene']}")
+    # differentiable batch: its first 2 scenes, all 32 substeps
+    cfgs, d, meta = workloads.batched_diff_scenes(max(4, int(128 * scale)))
+    m = 2
+    states = [orc.State(d["x"][i], d["v"][i], mass=np.full(len(d["x"][i]), cfgs[i].particle_mass),
+                        vol=np.full(len(d["x"][i]), cfgs[i].particle_volume),
+                        E=np.full(len(d["x"][i]), meta["e0"][i])) for i in range(m)]
+    ns = sum(len(d["x"][i]) for i in range(m))
+    tf, tb = diff_leg(cfgs[:m], states, meta["dt"], meta["substeps"],
+                      [lambda s: (2.0 / s.n) * s.x] * m)
+    gb = out.get("batched_adjoint", {})
+    legs["batched_diff_scenes"] = {
+        "forward_value": ns * meta["substeps"] / tf, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+        "adjoint_fwd_ratio": tb / tf,
+        "sample": f"first {m} scenes ({ns} particles) x {meta['substeps']} substeps: forward {tf:.2f} s, "
+                  f"reverse {tb:.2f} s",
+        "gpu_over_cpu_forward": gb["forward_value"] / (ns * meta["substeps"] / tf) if "forward_value" in gb else None}
+    log(f"cpu leg batched_diff_scenes: {legs['batched_diff_scenes']}")
+    return legs
+
+
 def run_batched(steps, warmup, scale, autotune=True):
     from molla_amd import MpmSimulation, workloads
 
@@ -189,6 +286,42 @@ def run_batched(steps, warmup, scale, autotune=True):
             "particles": n, "substeps": steps, "ms_per_step": 1e3 * el / steps,
             "hbm_frac": n * steps / el * BYTES_FWD / (HBM_PEAK_GBS * 1e9),
             "grids": [meta["grid_min"], meta["grid_max"]], "graph": True, "autotuned": tuned}
+
+
+def run_plain_per_scene(steps, scale):
+    """The same scenes as the batch, each stepped through its own
+    single-scene handle (the reference's one MpmSimulation per scene,
+    mpm.rs:2252-2353): every handle's step_n of `steps` substeps is issued in
+    turn and all are waited for at the end.  Launch sizes are the library
+    defaults (the autotuner's trials would cost more than the run)."""
+    from molla_amd import MpmSimulation, workloads
+
+    n_scenes = max(8, int(512 * scale))
+    cfgs, xs, meta = workloads.batched_scenes(n_scenes)
+    dt = meta["dt"]
+    sims = []
+    try:
+        for c, x in zip(cfgs, xs):
+            s = MpmSimulation(c, max_particles=len(x))
+            s.set_particles(x)
+            sims.append(s)
+        for s in sims:  # warm-up: first sort, grid allocation
+            s.step_n(dt, 2)
+        for s in sims:
+            s.synchronize()
+        t0 = time.perf_counter()
+        for s in sims:
+            s.step_n(dt, steps)
+        for s in sims:
+            s.synchronize()
+        el = time.perf_counter() - t0
+    finally:
+        for s in sims:
+            s.close()
+    n = meta["particles"]
+    return {"workload": "batched_scenes, one handle per scene", "scenes": n_scenes, "particles": n,
+            "substeps": steps, "value": n * steps / el, "unit": "particle-substeps/s",
+            "ms_per_step": 1e3 * el / steps}
 
 
 def run_plain(scale):
@@ -238,6 +371,53 @@ def timed_diff(sim, dt, steps, every, loss_grad):
                   "adjoint_fwd_ratio_wall": t_bwd / t_fwd}
 
 
+def device_wall(sim, dt, steps, every, times, reset, loss_grad):
+    """Adds the wall-clock ratio with the gradients kept in HBM
+    (diff_backward_device) to `times`; the host-transfer variant's wall ratio
+    is kept as adjoint_fwd_ratio_wall_host.  A failure is reported, not
+    hidden."""
+    times = dict(times)
+    times["adjoint_fwd_ratio_wall_host"] = times.pop("adjoint_fwd_ratio_wall")
+    times["forward_wall_ms_host"] = times.pop("forward_wall_ms")
+    times["backward_wall_ms_host"] = times.pop("backward_wall_ms")
+    try:
+        reset()
+        timed_diff_device(sim, dt, steps, every, loss_grad)  # warm-up
+        reset()
+        times.update(timed_diff_device(sim, dt, steps, every, loss_grad))
+    except Exception as e:  # reported, never silently replaced
+        times["device_wall_error"] = str(e)
+    return times
+
+
+def timed_diff_device(sim, dt, steps, every, loss_grad):
+    """One timed diff_forward + diff_backward_device: the reverse pass reads
+    dJ/dx_T from device memory and writes the initial-state gradients to
+    device memory, so nothing crosses PCIe inside it.  dJ/dx_T is placed in
+    HBM before the timed call (a caller whose loss runs on the GPU has it
+    there).  Wall-clock times of the two calls, each up to the end of its
+    device work."""
+    from molla_amd.devmem import DeviceArray
+
+    n = sim.n
+    outs = [DeviceArray((n, 3)), DeviceArray((n, 3)), DeviceArray(n), DeviceArray(2, np.float64)]
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.diff_forward(dt, steps, every)
+    sim.synchronize()
+    t_fwd = time.perf_counter() - t0
+    _, gx = loss_grad()
+    gx_dev = DeviceArray.from_host(gx)
+    t0 = time.perf_counter()
+    sim.diff_backward_device(gx_dev.address, 0, *(o.address for o in outs))
+    sim.synchronize()
+    t_bwd = time.perf_counter() - t0
+    for o in (*outs, gx_dev):
+        o.free()
+    return {"forward_wall_ms": 1e3 * t_fwd, "backward_wall_ms": 1e3 * t_bwd,
+            "adjoint_fwd_ratio_wall": t_bwd / t_fwd}
+
+
 def run_adjoint(scale):
     """Forward with checkpoints + reverse pass of the 65,536-particle diff scene
     (64 substeps, checkpoint every 16, COM loss); reports the reverse/forward
@@ -264,6 +444,8 @@ def run_adjoint(scale):
     sim.diff_backward(loss_grad()[1])
     reset()
     J, g, times = timed_diff(sim, dt, steps, every, loss_grad)
+    reset()
+    times = device_wall(sim, dt, steps, every, times, reset, loss_grad)
     reset()
     sim.synchronize()
     t0 = time.perf_counter()
@@ -306,6 +488,7 @@ def run_batched_adjoint(scale):
     sim.diff_backward(loss_grad()[1])
     start(meta["e0"])
     J, g, times = timed_diff(sim, dt, steps, every, loss_grad)
+    times = device_wall(sim, dt, steps, every, times, lambda: start(meta["e0"]), loss_grad)
     sim.close()
     ge = g["youngs_per_scene"]
     # the loss is minimised at E_true: dJ/dE should point from E0 towards it
@@ -359,9 +542,21 @@ def main():
     log(f"autotuned: {tuned}")
     if args.warmup:
         time_steps(sim, dt, args.warmup, args.graph)
-    el = time_steps(sim, dt, args.steps, args.graph)
+    sorts0 = sim.read_status()["sorts"]
+    el_raw = time_steps(sim, dt, args.steps, args.graph)
+    sorts_in = sim.read_status()["sorts"] - sorts0
+    # The layout sort runs every `sort_every` substeps; a timed window shorter
+    # than that interval may hold none.  The window's time is adjusted to carry
+    # exactly steps / sort_every sorts, each at the time of a forced sort
+    # measured right after.
+    every = int((tuned or {}).get("sort_every", 16))  # the library's default interval is 16
+    t_sort = time_sort(sim, dt, args.graph)
+    amort = args.steps / every if every else 0.0
+    el = el_raw + (amort - sorts_in) * t_sort
+    sim.set_sort_interval(every)
     value = n * args.steps / el
-    log(f"timed {args.steps} substeps: {el * 1e3:.2f} ms -> {value:.3e} particle-substeps/s")
+    log(f"timed {args.steps} substeps: {el_raw * 1e3:.2f} ms with {sorts_in} sorts; amortised sorts "
+        f"({amort:.3f} x {t_sort * 1e3:.3f} ms) -> {el * 1e3:.2f} ms -> {value:.3e} particle-substeps/s")
     # The same number of substeps again with HIP events on the library's
     # stream around every launch, for the per-kernel durations (the events
     # cost about 5 % of a substep, so `value` comes from the run above; graph
@@ -395,6 +590,9 @@ def main():
 
     out = {"metric": METRIC, "value": value, "unit": "particle-substeps/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el * 1e3 / args.steps,
+           "sort_amortised": {"sort_every": every, "sort_ms": t_sort * 1e3, "sorts_in_window": sorts_in,
+                              "sorts_carried": amort, "window_ms": el_raw * 1e3,
+                              "value_window_only": n * args.steps / el_raw},
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic",
            "config": {"workload": "large_scene", "particles": n, "grid": meta["grid"], "dt": dt,
@@ -414,10 +612,15 @@ def main():
             raise RuntimeError("skipped (--no-plain)")
         out["plain"] = run_plain(args.scale)
         log(f"plain: {out['plain']}")
+        out["plain_per_scene"] = run_plain_per_scene(min(args.steps, 100), args.scale)
+        log(f"plain per scene: {out['plain_per_scene']}")
         if "batched" in out and "value" in out["batched"]:
-            out["batched"]["vs_plain"] = out["batched"]["value"] / out["plain"]["value"]
+            # the batch against the same scenes stepped one handle each
+            out["batched"]["vs_plain"] = out["batched"]["value"] / out["plain_per_scene"]["value"]
+            out["batched"]["vs_plain_cube"] = out["batched"]["value"] / out["plain"]["value"]
     except Exception as e:  # reported, never silently replaced
-        out["plain"] = {"error": str(e)}
+        out.setdefault("plain", {"error": str(e)})
+        out.setdefault("plain_per_scene", {"error": str(e)})
 
     if not args.no_adjoint:
         try:
@@ -452,6 +655,10 @@ def main():
                                     "sample": f"{sample}, f32 threaded port of the oracle, "
                                               f"{cb['fair_seconds']:.2f} s",
                                     "rel_error_vs_fp64": cb["fair_error"]}
+        try:
+            out["cpu_legs"] = cpu_legs(args.scale, out)
+        except Exception as e:  # reported, never silently replaced
+            out["cpu_legs"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -138,6 +138,12 @@ typedef enum {
 } mpm_field;
 int mpm_write_field(mpm_sim* sim, int field, const void* data, uint32_t n);
 int mpm_read_field(mpm_sim* sim, int field, void* out, uint32_t n);
+/* No counterpart in the reference (its reads return host Vecs).  The same
+ * field into a caller-owned DEVICE buffer of n x components words, in
+ * original particle order, ordered on the handle's stream with no host
+ * synchronisation: for callers that keep the state in HBM (a loss and its
+ * gradient computed on the GPU, as for mpm_diff_backward_device). */
+int mpm_read_field_device(mpm_sim* sim, int field, void* out_dev, uint32_t n);
 /* read_positions_with / read_velocities_with (mpm.rs:2718-2790). */
 int mpm_read_positions(mpm_sim* sim, float* out, uint32_t n);
 int mpm_read_velocities(mpm_sim* sim, float* out, uint32_t n);
@@ -227,6 +233,14 @@ int mpm_diff_forward(mpm_sim* sim, float dt, uint32_t steps, uint32_t checkpoint
  * (mpm_adjoint.rs:76-113). */
 int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad_v_final,
                       float* grad_x0, float* grad_v0, float* grad_E, double* grad_mu_lambda);
+/* No counterpart in the reference, whose adjoint is host-only
+ * (mpm_adjoint.rs:121-137).  mpm_diff_backward with every array a DEVICE
+ * pointer (same shapes, original particle order): the gradients stay in HBM
+ * and nothing crosses PCIe; the work is ordered on the handle's stream
+ * (mpm_synchronize waits for the outputs). */
+int mpm_diff_backward_device(mpm_sim* sim, const float* grad_x_final_dev, const float* grad_v_final_dev,
+                             float* grad_x0_dev, float* grad_v0_dev, float* grad_E_dev,
+                             double* grad_mu_lambda_dev);
 /* The rest of the StateAdjoint of the last backward (mpm_adjoint.rs:76-113):
  * dJ/d(initial field) in original particle order for field POSITION,
  * VELOCITY, C, F (row-major 3x3) or YOUNGS. */

@@ -684,7 +684,8 @@ int create(const mpm_scene_config* cfgs, uint32_t n_scenes, mpm_sim** out) {
         s->pbase.push_back(uint32_t(cap));
         cap += cfgs[i].max_particles;
     }
-    if (tiles >= (1ull << 26) || cap >= (1ull << 31) || cap == 0) {
+    // kernels address a particle plane by a 32-bit byte offset (4 slot)
+    if (tiles >= (1ull << 26) || cap >= (1ull << 30) || cap == 0) {
         *out = s;
         return fail(s, MPM_ERR_INVALID, "grid or particle capacity too large");
     }
@@ -1144,7 +1145,8 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
     return MPM_OK;
 }
 
-int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
+// `device`: gx and gv are device pointers (the caller's HBM), else host arrays.
+int diff_backward(mpm_sim* s, const float* gx, const float* gv, bool device = false) {
     if (s->diff_steps == 0) return fail(s, MPM_ERR_INVALID, "mpm_diff_forward must run first");
     if (s->layout_gen != s->diff_gen || s->sorts != s->diff_sorts || s->ps_cur != s->diff_ps)
         return fail(s, MPM_ERR_INVALID,
@@ -1168,9 +1170,15 @@ int diff_backward(mpm_sim* s, const float* gx, const float* gv) {
     s->adj_valid = false;
     // the final-state gradients to the staging buffer, in the caller's order
     const size_t vec = (size_t)s->n * 12;
-    std::memcpy(s->hpin, gx, vec);
-    if (gv) std::memcpy(static_cast<char*>(s->hpin) + vec, gv, vec);
-    HIP_TRY(s, hipMemcpyAsync(s->staging, s->hpin, gv ? 2 * vec : vec, hipMemcpyHostToDevice, s->stream));
+    if (device) {
+        HIP_TRY(s, hipMemcpyAsync(s->staging, gx, vec, hipMemcpyDeviceToDevice, s->stream));
+        if (gv)
+            HIP_TRY(s, hipMemcpyAsync(s->staging + 3 * (size_t)s->n, gv, vec, hipMemcpyDeviceToDevice, s->stream));
+    } else {
+        std::memcpy(s->hpin, gx, vec);
+        if (gv) std::memcpy(static_cast<char*>(s->hpin) + vec, gv, vec);
+        HIP_TRY(s, hipMemcpyAsync(s->staging, s->hpin, gv ? 2 * vec : vec, hipMemcpyHostToDevice, s->stream));
+    }
     Stage span(s, MPM_STAGE_DIFF_BACKWARD);
     rc = diff_backward_pass(s, gv != nullptr, false, span);
     if (rc == BACKWARD_REPLAN) rc = diff_backward_pass(s, gv != nullptr, true, span);
@@ -1201,14 +1209,16 @@ int read_adjoint(mpm_sim* s, int field, float* out) {
 
 // The outputs of a backward in one transfer: the requested adjoint planes in
 // original order, then the fixed-order sum of the per-chunk (mu, lambda).
+struct Out3 {
+    float* dst;
+    int comp, nc;
+};
+
 int read_backward_outputs(mpm_sim* s, float* gx0, float* gv0, float* gE, double* gml) {
     const size_t n = s->n;
-    struct Out {
-        float* dst;
-        int comp, nc;
-    } outs[3] = {{gx0, mpm::AX, 3}, {gv0, mpm::AV, 3}, {gE, mpm::AE, 1}};
+    const Out3 outs[3] = {{gx0, mpm::AX, 3}, {gv0, mpm::AV, 3}, {gE, mpm::AE, 1}};
     size_t off = 0;
-    for (const Out& o : outs) {
+    for (const Out3& o : outs) {
         if (!o.dst) continue;
         HIP_TRY(s, mpm::launch_unpermute(s->adj, s->cap, o.comp, o.nc, s->slot_of_id, s->staging + off, s->n,
                                          s->stream));
@@ -1224,12 +1234,22 @@ int read_backward_outputs(mpm_sim* s, float* gx0, float* gv0, float* gE, double*
     }
     HIP_TRY(s, hipStreamSynchronize(s->stream));
     off = 0;
-    for (const Out& o : outs) {
+    for (const Out3& o : outs) {
         if (!o.dst) continue;
         std::memcpy(o.dst, pin + off * 4, n * o.nc * 4);
         off += n * o.nc;
     }
     if (gml) std::memcpy(gml, pin + dbl, 2 * sizeof(double));
+    return MPM_OK;
+}
+
+// The outputs of a backward straight into the caller's device buffers.
+int read_backward_outputs_device(mpm_sim* s, float* gx0, float* gv0, float* gE, double* gml) {
+    const Out3 outs[3] = {{gx0, mpm::AX, 3}, {gv0, mpm::AV, 3}, {gE, mpm::AE, 1}};
+    for (const Out3& o : outs)
+        if (o.dst)
+            HIP_TRY(s, mpm::launch_unpermute(s->adj, s->cap, o.comp, o.nc, s->slot_of_id, o.dst, s->n, s->stream));
+    if (gml) HIP_TRY(s, mpm::launch_sum_pairs(s->gml, s->status + mpm::ST_NCHUNKS, s->max_schunks, gml, s->stream));
     return MPM_OK;
 }
 
@@ -1407,6 +1427,18 @@ int mpm_read_field(mpm_sim* sim, int field, void* out, uint32_t n) {
     HIP_TRY(sim, hipMemcpyAsync(out, sim->staging, (size_t)n * nc * 4, hipMemcpyDeviceToHost, sim->stream));
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
     flush_profile(sim);
+    return MPM_OK;
+}
+
+int mpm_read_field_device(mpm_sim* sim, int field, void* out_dev, uint32_t n) {
+    if (!sim) return MPM_ERR_INVALID;
+    int c0, nc;
+    if (field_layout(field, &c0, &nc) != 0) return fail(sim, MPM_ERR_INVALID, "unknown field");
+    if (n != sim->n) return fail(sim, MPM_ERR_INVALID, "output length must equal the particle count");
+    if (n == 0) return MPM_OK;
+    if (!out_dev) return fail(sim, MPM_ERR_INVALID, "out must not be NULL");
+    HIP_TRY(sim, mpm::launch_unpermute(sim->ps[sim->ps_cur], sim->cap, c0, nc, sim->slot_of_id,
+                                       static_cast<float*>(out_dev), n, sim->stream));
     return MPM_OK;
 }
 
@@ -1684,6 +1716,14 @@ int mpm_diff_backward(mpm_sim* sim, const float* grad_x_final, const float* grad
     int rc = diff_backward(sim, grad_x_final, grad_v_final);
     if (rc != MPM_OK) return rc;
     return read_backward_outputs(sim, grad_x0, grad_v0, grad_E, grad_mu_lambda);
+}
+
+int mpm_diff_backward_device(mpm_sim* sim, const float* grad_x_final_dev, const float* grad_v_final_dev,
+                             float* grad_x0_dev, float* grad_v0_dev, float* grad_E_dev, double* grad_mu_lambda_dev) {
+    if (!sim) return MPM_ERR_INVALID;
+    int rc = diff_backward(sim, grad_x_final_dev, grad_v_final_dev, true);
+    if (rc != MPM_OK) return rc;
+    return read_backward_outputs_device(sim, grad_x0_dev, grad_v0_dev, grad_E_dev, grad_mu_lambda_dev);
 }
 
 int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,

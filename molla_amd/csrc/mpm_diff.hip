@@ -61,8 +61,8 @@ __device__ __forceinline__ double warp_sum(double v) {
 // gx, gv, gC, gF are dJ/d(post-state); on return gx holds the G2P part of
 // dJ/dx (plus the pass-through), gF = (I + dt C')^T dJ/dF', and sp the
 // scatter of dJ/d(grid v).
-__device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneDev& sc, uint32_t slot, float* gx,
-                                                 const float* gv, const float* gC, float* gF, Scatter& sp) {
+__device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneDev& sc, const Chunk& ch, uint32_t slot,
+                                                 float* gx, const float* gv, const float* gC, float* gF, Scatter& sp) {
     const uint32_t cap = a.cap;
     const float* __restrict__ ps = a.ps;
     const float dt = a.dt;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneD
     for (int k = 0; k < 9; ++k) sp.A[k] = B[k];
     sp.m = 0.f;
     sp.on = true;
-    sp.esc = stencil_escapes(a, sc, slot, sp.b);
+    sp.esc = stencil_escapes(a, sc, ch, slot, sp.b);
 }
 
 // Reverse of P2G, of the stress and of the SVD for the particle in `slot`
@@ -451,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 gC[k] = adj[(size_t)(AC + k) * cap + slot];
                 gF[k] = adj[(size_t)(AF + k) * cap + slot];
             }
-            adj_g2p_particle(a, sc, slot, gx, gv, gC, gF, sp);
+            adj_g2p_particle(a, sc, ch, slot, gx, gv, gC, gF, sp);
 #pragma unroll
             for (int k = 0; k < 3; ++k) adj[(size_t)(AX + k) * cap + slot] = gx[k];
 #pragma unroll
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2))) v
         sp.on = false;
         sp.esc = false;
         if (active) {
-            adj_g2p_particle(ap, sc, slot, gx, gv, gC, gF, sp);
+            adj_g2p_particle(ap, sc, ch, slot, gx, gv, gC, gF, sp);
 #pragma unroll
             for (int k = 0; k < 3; ++k) adj[(size_t)(AX + k) * cap + slot] = gx[k];
 #pragma unroll

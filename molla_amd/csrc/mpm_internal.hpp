@@ -67,8 +67,13 @@ struct Chunk {
     uint32_t scene;
     uint32_t start;
     uint32_t count;
-    uint32_t pad;
+    uint32_t pad;  // chunks of the sorted layout: chunk_row_tag of their tile row; else 0
 };
+
+// Chunks of the sorted layout lie in one tile row (ty, tz) of their scene.
+__host__ __device__ inline uint32_t chunk_row_tag(uint32_t ty, uint32_t tz) {
+    return 0x80000000u | ty | (tz << 15);
+}
 
 constexpr uint32_t CHUNK_PARTICLES = 256;  // one particle per thread of a block
 
@@ -89,8 +94,9 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 // tile lower-bound table of the last sort and sums their partials per cell.
 // A particle whose stencil has left that neighbourhood since the sort is
 // scattered with global atomics instead (and counted in ST_ESCAPES).
-// 1600 cells of four int64 channels (51 KB) lets three blocks share a CU's
-// 160 KB LDS; a chunk's box (4 or 8 tiles of one tile row plus the stencil
+// 1600 cells of three 64-bit words (38 KB; the differentiable model's four
+// int64 channels: 51 KB) let three blocks share a CU's 160 KB LDS; a chunk's
+// box (4 or 8 tiles of one tile row plus the stencil
 // reach and the motion since the last sort) stays well inside it, and a
 // larger box falls back to global atomics (counted in ST_FALLBACK).
 constexpr int LDS_CELLS = 1600;

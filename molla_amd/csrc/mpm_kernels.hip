@@ -81,6 +81,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
         const uint32_t slot = ch.start + threadIdx.x;
+        const uint32_t boff = 4u * slot;
         const bool active = threadIdx.x < ch.count;
         // Gather staging: the chunk's P2G box of the previous pass holds every
         // stencil this gather reads (same particles, same positions), so the
@@ -95,17 +96,17 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         if (active) {
             float x[3], F[9];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) x[k] = ps[(size_t)(CX + k) * cap + slot];
+            for (int k = 0; k < 3; ++k) x[k] = plane_ld(ps, cap, CX + k, boff);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) F[k] = ps[(size_t)(CF + k) * cap + slot];
-            float jp = ps[(size_t)CJP * cap + slot];
-            const float m = ps[(size_t)CMASS * cap + slot];
-            const float vol = ps[(size_t)CVOL * cap + slot];
-            const uint32_t matid = __float_as_uint(ps[(size_t)CMAT * cap + slot]);
+            for (int k = 0; k < 9; ++k) F[k] = plane_ld(ps, cap, CF + k, boff);
+            float jp = plane_ld(ps, cap, CJP, boff);
+            const float m = plane_ld(ps, cap, CMASS, boff);
+            const float vol = plane_ld(ps, cap, CVOL, boff);
+            const uint32_t matid = __float_as_uint(plane_ld(ps, cap, CMAT, boff));
             const float* mrow = mat_row(a, sc, matid);
             float mu = sc.mu, lam = sc.lam;
             if (a.has_E) {
-                float E = ps[(size_t)CE * cap + slot];
+                float E = plane_ld(ps, cap, CE, boff);
                 mu = E / (2.0f * (1.0f + sc.poisson));
                 lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
             }
@@ -113,9 +114,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
             float v[3], C[9];
             if (MODE == MODE_P2G) {
 #pragma unroll
-                for (int k = 0; k < 3; ++k) v[k] = ps[(size_t)(CV + k) * cap + slot];
+                for (int k = 0; k < 3; ++k) v[k] = plane_ld(ps, cap, CV + k, boff);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) C[k] = ps[(size_t)(CC + k) * cap + slot];
+                for (int k = 0; k < 9; ++k) C[k] = plane_ld(ps, cap, CC + k, boff);
                 if (MODEL == MODEL_SHADER) {
                     float fa = finite_acc(0.0f, jp);
 #pragma unroll
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                     affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp, dt, vol, m, C, sp.A);
                     if (a.aff_out)
 #pragma unroll
-                        for (int k = 0; k < 9; ++k) a.aff_out[(size_t)k * cap + slot] = sp.A[k];
+                        for (int k = 0; k < 9; ++k) plane_st(a.aff_out, cap, k, boff, sp.A[k]);
                 }
             } else {
                 // ---- G2P (mpm.wgsl:1241-1353 / mpm_cpu_diff.rs:544-580) ----
@@ -191,13 +192,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                                         ri[2] += wi * g.z;
                                     }
                                 }
+                                // j and k are unrolled constants: rows with j = 0 or
+                                // k = 0 add nothing to those moments
                                 const float wj = float(j) * wjk, wk = float(k) * wjk;
 #pragma unroll
                                 for (int r = 0; r < 3; ++r) {
                                     vn[r] += wjk * rs[r];
                                     M[3 * r] += wjk * ri[r];
-                                    M[3 * r + 1] += wj * rs[r];
-                                    M[3 * r + 2] += wk * rs[r];
+                                    if (j > 0) M[3 * r + 1] += wj * rs[r];
+                                    if (k > 0) M[3 * r + 2] += wk * rs[r];
                                 }
                             }
                         }
@@ -364,23 +367,23 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                         }
                         // ---- write the particle state ----
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) po[(size_t)(CX + k) * cap + slot] = nx[k];
+                        for (int k = 0; k < 3; ++k) plane_st(po, cap, CX + k, boff, nx[k]);
                         if (a.write_vc || MODE == MODE_G2P) {
 #pragma unroll
-                            for (int k = 0; k < 3; ++k) po[(size_t)(CV + k) * cap + slot] = vn[k];
+                            for (int k = 0; k < 3; ++k) plane_st(po, cap, CV + k, boff, vn[k]);
 #pragma unroll
-                            for (int k = 0; k < 9; ++k) po[(size_t)(CC + k) * cap + slot] = Cn[k];
+                            for (int k = 0; k < 9; ++k) plane_st(po, cap, CC + k, boff, Cn[k]);
                         }
 #pragma unroll
-                        for (int k = 0; k < 9; ++k) po[(size_t)(CF + k) * cap + slot] = Fn[k];
-                        if (MODEL == MODEL_SHADER && plast_ok) po[(size_t)CJP * cap + slot] = jp_new;
+                        for (int k = 0; k < 9; ++k) plane_st(po, cap, CF + k, boff, Fn[k]);
+                        if (MODEL == MODEL_SHADER && plast_ok) plane_st(po, cap, CJP, boff, jp_new);
                         if (MODE == MODE_G2P2G && plast_ok) {
                             if (MODEL == MODEL_CPUDIFF) svd3<MODEL_CPUDIFF>(Fn, d);
                             (void)have_svd;
                             affine_from_svd<MODEL>(sc, mrow, d, mu, lam, jp_new, dt, vol, m, Cn, sp.A);
                             if (a.aff_out)
 #pragma unroll
-                                for (int k = 0; k < 9; ++k) a.aff_out[(size_t)k * cap + slot] = sp.A[k];
+                                for (int k = 0; k < 9; ++k) plane_st(a.aff_out, cap, k, boff, sp.A[k]);
 #pragma unroll
                             for (int k = 0; k < 3; ++k) {
                                 x[k] = nx[k];
@@ -404,7 +407,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 sp.m = m;
                 sp.on = true;
                 sp.esc = !a.use_lds;
-                if (a.use_lds && stencil_escapes(a, sc, slot, sp.b)) sp.esc = true;
+                if (a.use_lds && stencil_escapes(a, sc, ch, slot, sp.b)) sp.esc = true;
             }
         }
         if (MODE != MODE_G2P) block_scatter<true, WIDE>(a, sc, sp, ci, L);

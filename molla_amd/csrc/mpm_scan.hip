@@ -171,7 +171,9 @@ __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t
         ch.scene = scn;
         ch.start = j;
         ch.count = min((uint32_t)CHUNK_PARTICLES, seg_end - j);
-        ch.pad = 0;
+        // the tile row of the chunk's segment (read by stencil_escapes)
+        const uint32_t r = (key - sc.tile_base) / sc.W;
+        ch.pad = chunk_row_tag(r % sc.W, r / sc.W);
         chunks[c] = ch;
     }
 }

@@ -30,6 +30,17 @@ __device__ __forceinline__ ChunkWalk chunk_walk(uint32_t nch) {
     return {lo + blockIdx.x / NUM_XCD, hi, g / NUM_XCD};
 }
 
+// Plane c of a particle array at one lane's slot, as the wave-uniform plane
+// base plus the slot's 32-bit byte offset `boff` = 4 slot (capacities stay
+// below 2^30 particles, create() enforces it), which lets the compiler use
+// the SGPR-base form of global loads and stores.
+__device__ __forceinline__ float plane_ld(const float* ps, uint32_t cap, int c, uint32_t boff) {
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(ps + (size_t)c * cap) + boff);
+}
+__device__ __forceinline__ void plane_st(float* ps, uint32_t cap, int c, uint32_t boff, float v) {
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(ps + (size_t)c * cap) + boff) = v;
+}
+
 __device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int iy, int iz) {
     uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
     return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
@@ -110,23 +121,29 @@ struct Scatter {
     float A[9];  // m C + 4/dx^2 stress
 };
 
-// LDS workspace of the block scatter.  The block's box is accumulated as
-// 32-bit fixed point: on gfx950 LDS integer atomics run at about the LDS
-// read rate while LDS f32 atomic adds are ~20x slower, and integer sums are
-// independent of the order in which lanes and waves arrive.
-//   narrow (int32): each contribution is rounded to 2^-23 of the power of
-//     two above the block's bound on its values -- about the precision of f32
-//     accumulation for the forward P2G, whose contributions in a block are of
-//     one magnitude.  Two channels share one 64-bit word per cell, so one
-//     ds_add_u64 adds both (see pack_lo_hi);
-//   wide (int64): each f32 contribution converts exactly (2^-50 of the
-//     largest), for sums whose terms span many magnitudes and cancel (the
-//     adjoint's dJ/d grid v).
+// LDS workspace of the block scatter.  The block's box is accumulated in
+// fixed point: on gfx950 LDS integer atomics run at about the LDS read rate
+// while LDS f32 atomic adds are ~20x slower, and integer sums are independent
+// of the order in which lanes and waves arrive.
+//   narrow (the shader model's forward P2G): three 64-bit words per cell,
+//     each added with one ds_add_u64 of raw conversion bit patterns (no
+//     per-contribution offset removal; see NarrowWords):
+//       acc[0]  mass as int64, 2^-50 of the block's largest particle mass, so
+//               a cell's mass keeps f32-like relative precision however small
+//               it is (the grid update's m <= 1e-9 test, mpm.wgsl:1019, is
+//               then decided as in f32 arithmetic);
+//       acc[1]  (px, py) as two int32 halves, 2^-23 of the block's bound on
+//               any single momentum value;
+//       acc[2]  (pz, 1): pz as int32 plus a count of contributions, which
+//               removes the conversion offsets from all three words;
+//   wide (int64, four words): each f32 contribution converts exactly (2^-50
+//     of the largest), for sums whose terms span many magnitudes and cancel
+//     (the adjoint's dJ/d grid v, the differentiable model's P2G).
 template <bool WIDE>
 struct ScatterLdsT {
-    // (m, px, py, pz); before the scatter the same space stages the gather's
-    // grid velocities (LDS_CELLS float4)
-    alignas(16) typename std::conditional<WIDE, unsigned long long, int>::type acc[4][LDS_CELLS];
+    // before the scatter the same space stages the gather's grid velocities
+    // (LDS_CELLS float4)
+    alignas(16) unsigned long long acc[WIDE ? 4 : 3][LDS_CELLS];
     int red[WAVES * 6];
     float redf[WAVES * 2];
     int box[16];
@@ -134,27 +151,44 @@ struct ScatterLdsT {
 };
 using ScatterLds = ScatterLdsT<false>;
 
-// Two int32 contributions as one 64-bit addend, lo + hi * 2^32 (mod 2^64).
-// Sums of such words give sum(lo) + sum(hi) * 2^32, from which both sums are
-// recovered exactly when each fits an int32 (unpack_lo_hi): the low half is
-// sum(lo) mod 2^32, and the rest is sum(hi).  The narrow scatter's sums of
-// 256 contributions below 2^22 each stay below 2^30.
-__device__ __forceinline__ unsigned long long pack_lo_hi(int lo, int hi) {
-    return (unsigned long long)(long long)lo + ((unsigned long long)(uint32_t)hi << 32);
-}
-__device__ __forceinline__ void unpack_lo_hi(unsigned long long s, int& lo, int& hi) {
-    lo = (int)(uint32_t)s;
-    hi = (int)(uint32_t)((s - (unsigned long long)(long long)lo) >> 32);
+// The narrow scatter's words.  A momentum value q (|w q| < 2^22 after the
+// scale) converts with one FMA: the f32 bits of w q + 1.5 * 2^23 are
+// MAGIC32 + round(w q).  The mass converts with one f64 FMA: the f64 bits of
+// w m + 1.5 * 2^52 are MAGIC64 + round(w m) for 0 <= w m < 2^51.  A cell's
+// sums of N such terms carry N times each offset; the third word's high half
+// adds 1 per term, so
+//   acc[2] = N (2^32 + MAGIC32) + sum(pz),  |sum(pz)| < 2^30 < (2^32 + MAGIC32) / 2,
+// gives N by rounding, and with it the exact sums of all three words.
+struct NarrowWords {
+    static constexpr uint32_t MAGIC32 = 0x4B400000u;                 // bits of 1.5 * 2^23
+    static constexpr unsigned long long MAGIC64 = 0x4338000000000000ull;  // bits of 1.5 * 2^52
+    static constexpr double K = 4294967296.0 + 1262485504.0;        // 2^32 + MAGIC32
+    __device__ static __forceinline__ void unpack(unsigned long long wm, unsigned long long wxy,
+                                                  unsigned long long wz, long long& m, int& x, int& y, int& z) {
+        const unsigned long long n = (unsigned long long)rint((double)wz * (1.0 / K));
+        z = (int)(long long)(wz - n * ((1ull << 32) + MAGIC32));
+        m = (long long)(wm - n * MAGIC64);
+        unpack_lo_hi_raw(wxy - n * ((unsigned long long)MAGIC32 * ((1ull << 32) + 1ull)), x, y);
+    }
+    // the two int32 halves of a 64-bit sum lo + hi * 2^32 (each |.| < 2^31)
+    __device__ static __forceinline__ void unpack_lo_hi_raw(unsigned long long s, int& lo, int& hi) {
+        lo = (int)(uint32_t)s;
+        hi = (int)(uint32_t)((s - (unsigned long long)(long long)lo) >> 32);
+    }
+};
+
+// round(a * b) + MAGIC32 as u32 bits for |a * b| < 2^22.
+__device__ __forceinline__ uint32_t raw32(float a, float b) {
+    return __float_as_uint(__builtin_fmaf(a, b, 12582912.0f));
 }
 
-// round(a * b) as an int32 for |a * b| < 2^22: one f32 FMA with 1.5 * 2^23
-// rounds the product once and leaves the integer in the low mantissa bits.
-__device__ __forceinline__ int fixed32(float a, float b) {
-    return __float_as_int(__builtin_fmaf(a, b, 12582912.0f)) - 0x4B400000;
+// round(a * b) + MAGIC64 as u64 bits for 0 <= a * b < 2^51 (one rounding).
+__device__ __forceinline__ unsigned long long raw64(double a, double b) {
+    return (unsigned long long)__double_as_longlong(__builtin_fma(a, b, 6755399441055744.0));
 }
 
-__device__ __forceinline__ void lds_add_i32(int* p, int v) {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+__device__ __forceinline__ unsigned long long pair32(uint32_t lo, uint32_t hi) {
+    return (unsigned long long)lo | ((unsigned long long)hi << 32);
 }
 
 // round(v * s) as a two's-complement int64 for |v * s| < 2^51: adding
@@ -265,12 +299,13 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             mp = fmaxf(mp, L.redf[w * 2 + 1]);
         }
         // A quadratic B-spline weight product is at most 0.75^3 < 0.422, so
-        // with values scaled below 2^23 every contribution w q stays below
-        // 0.422 * 2^23 < 2^22 (fixed32's range) and 256 of them below
-        // 0.85 * 2^31 (no int32 overflow).  Wide: values below 2^50, sums of
-        // 256 below 2^58.
+        // with narrow momentum values scaled below 2^23 every contribution w q
+        // stays below 0.422 * 2^23 < 2^22 (raw32's range) and 256 of them
+        // below 2^30.  Masses (both forms) and wide values: below 2^50, so
+        // every contribution is below 2^49 (raw64's range) and sums of 256
+        // below 2^57.
         const int top = WIDE ? 50 : 23;
-        L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, top - exp2_ceil(mm)) : 1.0;
+        L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, 50 - exp2_ceil(mm)) : 1.0;
         L.scale[1] = mp > 0.f && mp < 1e30f ? ldexp(1.0, top - exp2_ceil(mp)) : 1.0;
         const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
         int use = 2;  // nothing reduced in LDS
@@ -296,22 +331,15 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         const int x0 = L.box[0], y0 = L.box[1], z0 = L.box[2], bx = L.box[3], by = L.box[4];
         const int vol = bx * by * L.box[5];
         const double sm64 = L.scale[0], sp64 = L.scale[1];
-        // narrow: node values are pre-scaled and rounded by fixed32; wide:
+        // narrow: momentum values are pre-scaled and rounded by raw32; wide:
         // node values stay unscaled and fixed64 scales them exactly
-        const float sm = WIDE ? 1.0f : (float)sm64, sp = WIDE ? 1.0f : (float)sp64;
-        // narrow: the four int32 rows hold two 64-bit words per cell,
-        // (m, px) in pairs[i] and (py, pz) in pairs[LDS_CELLS + i]
-        unsigned long long* pairs = reinterpret_cast<unsigned long long*>(&L.acc[0][0]);
+        const float sp = WIDE ? 1.0f : (float)sp64;
         for (int i = tid; i < vol; i += BLOCK) {
-            if constexpr (WIDE) {
-                if (MASS) L.acc[0][i] = 0;
-                L.acc[1][i] = 0;
-                L.acc[2][i] = 0;
-                L.acc[3][i] = 0;
-            } else {
-                pairs[i] = 0ull;
-                pairs[LDS_CELLS + i] = 0ull;
-            }
+            if (WIDE && MASS) L.acc[0][i] = 0;
+            if (!WIDE) L.acc[0][i] = 0;
+            L.acc[1][i] = 0;
+            L.acc[2][i] = 0;
+            if constexpr (WIDE) L.acc[3][i] = 0;
         }
         __syncthreads();
         if (lds_on) {
@@ -330,45 +358,64 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                 a2[r] = dx * p.A[3 * r + 2] * sp;
                 c[r] = p.mv[r] * sp - (a0[r] * p.fx[0] + a1[r] * p.fx[1] + a2[r] * p.fx[2]);
             }
-            const float ms = p.m * sm;
+            // narrow mass: w0 * (w1 w2 m s) in f64, where the row factor is an
+            // exact product (24-bit by 24-bit) and raw64 rounds once
+            const double msd = (double)p.m * sm64;
+            const double w0d[3] = {(double)w[0][0], (double)w[0][1], (double)w[0][2]};
             const int G = int(sc.G);
             const int li0 = (p.b[0] - x0) + bx * ((p.b[1] - y0) + by * (p.b[2] - z0));
             // CHECK: test every node against the grid; a wave whose stencils
             // all lie inside (every particle after the position clamp) skips it
             auto nodes = [&](auto check) {
                 constexpr bool CHECK = decltype(check)::value;
+                // node values c + i a0 + j a1 + k a2, stepped by additions
+                float zk[3] = {c[0], c[1], c[2]};
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     const int iz = p.b[2] + k;
-                    if (CHECK && (iz < 0 || iz >= G)) continue;
+                    float yj[3] = {zk[0], zk[1], zk[2]};
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const int iy = p.b[1] + j;
-                        if (CHECK && (iy < 0 || iy >= G)) continue;
-                        const float wjk = w[1][j] * w[2][k];
-                        const int lrow = li0 + bx * (j + by * k);
-                        float q0 = c[0] + float(j) * a1[0] + float(k) * a2[0];
-                        float q1 = c[1] + float(j) * a1[1] + float(k) * a2[1];
-                        float q2 = c[2] + float(j) * a1[2] + float(k) * a2[2];
+                        const bool row_in = !CHECK || (iz >= 0 && iz < G && iy >= 0 && iy < G);
+                        if (row_in) {
+                            const float wjk = w[1][j] * w[2][k];
+                            const double wm = (double)wjk * msd;
+                            const int lrow = li0 + bx * (j + by * k);
+                            float q0 = yj[0], q1 = yj[1], q2 = yj[2];
 #pragma unroll
-                        for (int i = 0; i < 3; ++i) {
-                            const int ix = p.b[0] + i;
-                            if (!CHECK || (ix >= 0 && ix < G)) {
-                                const float wt = w[0][i] * wjk;
-                                if constexpr (WIDE) {
-                                    if (MASS) lds_add_u64(&L.acc[0][lrow + i], fixed64(wt * ms, sm64));
-                                    lds_add_u64(&L.acc[1][lrow + i], fixed64(wt * q0, sp64));
-                                    lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
-                                    lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
-                                } else {
-                                    lds_add_u64(&pairs[lrow + i], pack_lo_hi(MASS ? fixed32(wt, ms) : 0, fixed32(wt, q0)));
-                                    lds_add_u64(&pairs[LDS_CELLS + lrow + i], pack_lo_hi(fixed32(wt, q1), fixed32(wt, q2)));
+                            for (int i = 0; i < 3; ++i) {
+                                const int ix = p.b[0] + i;
+                                if (!CHECK || (ix >= 0 && ix < G)) {
+                                    const float wt = w[0][i] * wjk;
+                                    if constexpr (WIDE) {
+                                        if (MASS) lds_add_u64(&L.acc[0][lrow + i], fixed64(wt * p.m, sm64));
+                                        lds_add_u64(&L.acc[1][lrow + i], fixed64(wt * q0, sp64));
+                                        lds_add_u64(&L.acc[2][lrow + i], fixed64(wt * q1, sp64));
+                                        lds_add_u64(&L.acc[3][lrow + i], fixed64(wt * q2, sp64));
+                                    } else {
+                                        lds_add_u64(&L.acc[0][lrow + i], raw64(w0d[i], MASS ? wm : 0.0));
+                                        lds_add_u64(&L.acc[1][lrow + i], pair32(raw32(wt, q0), raw32(wt, q1)));
+                                        lds_add_u64(&L.acc[2][lrow + i], pair32(raw32(wt, q2), 1u));
+                                    }
+                                }
+                                if (i < 2) {
+                                    q0 += a0[0];
+                                    q1 += a0[1];
+                                    q2 += a0[2];
                                 }
                             }
-                            q0 += a0[0];
-                            q1 += a0[1];
-                            q2 += a0[2];
                         }
+                        if (j < 2) {
+                            yj[0] += a1[0];
+                            yj[1] += a1[1];
+                            yj[2] += a1[2];
+                        }
+                    }
+                    if (k < 2) {
+                        zk[0] += a2[0];
+                        zk[1] += a2[1];
+                        zk[2] += a2[2];
                     }
                 }
             };
@@ -388,13 +435,16 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                                       (float)((double)(long long)L.acc[2][i] * ip),
                                       (float)((double)(long long)L.acc[3][i] * ip));
         } else {
-            // int32 sums: one rounding to f32, then an exact power-of-two scale
-            const float im = (float)(1.0 / sm64), ip = (float)(1.0 / sp64);
+            // exact integer sums (NarrowWords): the mass through f64, the
+            // int32 momenta with one rounding to f32, then power-of-two scales
+            const double im = 1.0 / sm64;
+            const float ip = (float)(1.0 / sp64);
             for (int i = tid; i < vol; i += BLOCK) {
-                int sm, sx, sy, sz;
-                unpack_lo_hi(pairs[i], sm, sx);
-                unpack_lo_hi(pairs[LDS_CELLS + i], sy, sz);
-                part[i] = make_float4(MASS ? (float)sm * im : 0.f, (float)sx * ip, (float)sy * ip, (float)sz * ip);
+                long long sm;
+                int sx, sy, sz;
+                NarrowWords::unpack(L.acc[0][i], L.acc[1][i], L.acc[2][i], sm, sx, sy, sz);
+                part[i] = make_float4(MASS ? (float)((double)sm * im) : 0.f, (float)sx * ip, (float)sy * ip,
+                                      (float)sz * ip);
             }
         }
         __syncthreads();
@@ -407,11 +457,25 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
 
 // True when a stencil starting at cell `b` leaves the 3x3x3 tiles around the
 // particle's sorted tile, i.e. the chunk partials of the last sort cannot
-// hold its contribution.
-__device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDev& sc, uint32_t slot,
-                                                const int* b) {
-    const uint32_t tl = a.sorted_keys[slot] - sc.tile_base;
-    const int tk[3] = {int(tl % sc.W), int((tl / sc.W) % sc.W), int(tl / (sc.W * sc.W))};
+// hold its contribution.  The sorted chunk's tile row (ty, tz) comes with the
+// chunk, so the particle's tile x is its key minus the row's first key (no
+// division); a chunk without a row tag takes the division.
+__device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDev& sc, const Chunk& ch,
+                                                uint32_t slot, const int* b) {
+    const uint32_t key = a.sorted_keys[slot];
+    int tk[3];
+    if (ch.pad & 0x80000000u) {
+        tk[1] = int(ch.pad & 0x7fffu);
+        tk[2] = int((ch.pad >> 15) & 0xffffu);
+        const uint32_t tx = key - (sc.tile_base + sc.W * (uint32_t(tk[1]) + sc.W * uint32_t(tk[2])));
+        if (tx >= sc.W) return true;  // not in the chunk's row: keep it off the partials
+        tk[0] = int(tx);
+    } else {
+        const uint32_t tl = key - sc.tile_base;
+        tk[0] = int(tl % sc.W);
+        tk[1] = int((tl / sc.W) % sc.W);
+        tk[2] = int(tl / (sc.W * sc.W));
+    }
     bool esc = false;
 #pragma unroll
     for (int k = 0; k < 3; ++k)

@@ -110,6 +110,9 @@ _SIGS = {
     "mpm_read_status": (C.c_int, [C.c_void_p, C.POINTER(StatusInfo)]),
     "mpm_reset_status": (C.c_int, [C.c_void_p]),
     "mpm_sort_keys_debug": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mpm_read_field_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    "mpm_diff_backward_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
     "mpm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_read_profile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "mpm_reset_profile": (C.c_int, [C.c_void_p]),
@@ -279,6 +282,12 @@ class MpmSimulation:
         if nc == 9:
             return out.reshape(self.n, 3, 3)
         return out
+
+    def read_field_device(self, name: str, out_dev: int):
+        """The field into a caller-owned device buffer (address `out_dev`, n x
+        components words, original order); ordered on the handle's stream."""
+        fid, _ = FIELDS[name]
+        self._check(lib().mpm_read_field_device(self._h, fid, C.c_void_p(out_dev), self.n))
 
     def read_positions(self) -> np.ndarray:
         return self.read_field("position")
@@ -470,6 +479,16 @@ class MpmSimulation:
                             for i in range(len(self.counts))])
         return {"x": ox, "v": ov, "E": oe, "mu": float(ml[0]), "lambda": float(ml[1]),
                 "youngs": float(g_youngs), "youngs_per_scene": e_scene}
+
+    def diff_backward_device(self, grad_x_final_dev: int, grad_v_final_dev: int = 0, grad_x0_dev: int = 0,
+                             grad_v0_dev: int = 0, grad_E_dev: int = 0, grad_mu_lambda_dev: int = 0):
+        """diff_backward with device addresses for every array (0 = none):
+        the gradients stay in HBM; ordered on the handle's stream."""
+        def ptr(v):
+            return C.c_void_p(v) if v else None
+        self._check(lib().mpm_diff_backward_device(self._h, ptr(grad_x_final_dev), ptr(grad_v_final_dev),
+                                                   ptr(grad_x0_dev), ptr(grad_v0_dev), ptr(grad_E_dev),
+                                                   ptr(grad_mu_lambda_dev)))
 
     def read_adjoint(self, name: str):
         """dJ/d(initial field) of the last diff_backward, original particle order."""

@@ -10,7 +10,8 @@ recorded in the returned ``meta`` dict.
 * ``large_scene``    -- 2M particles on a 256^3 grid: half weakly-compressible
   fluid (bulk 1e4), half fixed-corotated elastic blocks (E = 5e4), dt = 5e-5.
 * ``batched_scenes`` -- 512 scenes, N ~ logU[1e3, 6.4e4], grid 32^3..96^3
-  matching extent, E ~ logU[1e3, 1e5], nu ~ U[0.1, 0.4].
+  from each scene's extent (block side x U[2.5, 4.5] + 8 cells),
+  E ~ logU[1e3, 1e5], nu ~ U[0.1, 0.4].
 * ``diff_scene``     -- 65,536 elastic particles on 64^3 for the adjoint.
 """
 from __future__ import annotations
@@ -93,11 +94,15 @@ def batched_scenes(n_scenes: int = 512, seed: int = 3, n_lo: float = 1e3, n_hi: 
                    dt: float = 1e-4) -> Tuple[List[MpmConfig], List[np.ndarray], Dict]:
     rng = np.random.default_rng(seed)
     counts = np.rint(np.exp(rng.uniform(math.log(n_lo), math.log(n_hi), n_scenes))).astype(int)
+    # Each scene's domain spans `extent` block sides plus an 8-cell margin,
+    # extent ~ U[2.5, 4.5] from a stream of its own (the other draws keep
+    # their values), so the grids cover 32^3..96^3.
+    extent = np.random.default_rng(seed + 7919).uniform(2.5, 4.5, n_scenes)
     cfgs, xs = [], []
     cfl = 0.0
     for i, N in enumerate(counts):
         a = (N / 8.0) ** (1.0 / 3.0)  # cube side in cells at 8 particles per cell
-        G = int(min(max(4 * math.ceil((2.5 * a + 8) / 4), g_lo), g_hi))
+        G = int(min(max(4 * math.ceil((extent[i] * a + 8) / 4), g_lo), g_hi))
         dx = 1.0 / G
         side = a * dx
         E = float(math.exp(rng.uniform(math.log(1e3), math.log(1e5))))
@@ -112,6 +117,7 @@ def batched_scenes(n_scenes: int = 512, seed: int = 3, n_lo: float = 1e3, n_hi: 
         cfl = max(cfl, dt * _cp(E, nu, rho) / dx)
     meta = {"workload": "batched_scenes", "scenes": n_scenes, "particles": int(counts.sum()),
             "dt": dt, "substeps": 500, "rho": rho, "seed": seed, "cfl_max": cfl,
+            "extent": extent.tolist(), "grids": [c.grid_size for c in cfgs],
             "grid_min": int(min(c.grid_size for c in cfgs)), "grid_max": int(max(c.grid_size for c in cfgs))}
     return cfgs, xs, meta
 

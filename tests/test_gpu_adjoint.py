@@ -137,6 +137,42 @@ def test_adjoint_matches_fp64_oracle():
     assert abs(float(np.sum(g["E"], dtype=np.float64)) - ge) <= 1e-3 * abs(ge) + 1e-12
 
 
+def test_device_backward_equals_host_backward():
+    # mpm_diff_backward_device keeps every array in HBM (caller-owned device
+    # buffers); fed the same dJ/dx_T it gives the host variant's gradients bit
+    # for bit, and the state read into device memory equals the host read.
+    from molla_amd.devmem import DeviceArray
+
+    cfg, d, meta = _scene(3000, seed=9)
+    d = _perturbed(d, 23)
+    n, dt, steps = len(d["x"]), meta["dt"], 12
+    target = np.array([0.52, 0.05, 0.49], np.float32)
+    host = _gpu_run(cfg, d, dt, steps, 4, np.zeros((n, 3), np.float32), None)  # x_T only
+    gx = ((2.0 / n) * (host["xT"] - target)).astype(np.float32)
+    ref = _gpu_run(cfg, d, dt, steps, 4, gx, None)
+    sim = MpmSimulation(cfg, max_particles=n)
+    sim.set_particles(d["x"], d["v"])
+    sim.write_field("youngs", d["E"])
+    sim.write_field("F", d["F"].reshape(-1, 9))
+    sim.diff_forward(dt, steps, 4)
+    xT = DeviceArray((n, 3))
+    sim.read_field_device("position", xT.address)
+    sim.synchronize()
+    assert np.array_equal(xT.download(), host["xT"])
+    gx_dev = DeviceArray.from_host(gx)
+    ox, ov, oe = DeviceArray((n, 3)), DeviceArray((n, 3)), DeviceArray(n)
+    ml = DeviceArray(2, np.float64)
+    sim.diff_backward_device(gx_dev.address, 0, ox.address, ov.address, oe.address, ml.address)
+    sim.synchronize()
+    sim.close()
+    assert np.array_equal(ox.download(), ref["x"])
+    assert np.array_equal(ov.download(), ref["v"])
+    assert np.array_equal(oe.download(), ref["E"])
+    assert ml.download().tolist() == [ref["mu"], ref["lambda"]]
+    for buf in (xT, gx_dev, ox, ov, oe, ml):
+        buf.free()
+
+
 def test_checkpoint_interval_does_not_change_gradients():
     cfg, d, meta = _scene(n=2048, seed=8)
     d = _perturbed(d, 22)
@@ -328,3 +364,43 @@ def test_profiling_spans_of_the_diff_calls():
     assert p["grid"][1] == 4 and p["diff_forward"][1] == 0
     sim.set_profiling(False)
     sim.close()
+
+
+def test_c4_gradients_match_fp64_finite_differences():
+    # The full differentiable scene (65,536 particles, 64^3, 64 substeps,
+    # checkpoint every 16, centroid loss mpm_adjoint.rs:516-556): the GPU
+    # adjoint against FP64 central differences of the oracle's forward on 32
+    # seeded parameters, 16 components of v0 and 16 per-particle E
+    # (tests/golden/c4_fd.json, tests/golden/make_c4_fd.py).
+    # Bound: |gpu - fd| <= 3e-3 |fd| + 4 |fd - fd(h/4)|.  The 3e-3 holds the
+    # F0 = I caveat: the reference's own adjoint from exactly F0 = I differs
+    # from the differences by up to 1.4e-3 in dJ/dv0 (recorded in the
+    # fixture as oracle_adjoint_F0_I; from F0 = I + 1e-9 N(0,1) it agrees to
+    # 2e-5), and the f32 state sits on one side or the other of that jump.
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c4_fd.json")) as f:
+        fx = json.load(f)
+    cfg, d, meta = workloads.diff_scene()
+    n, dt = len(d["x"]), meta["dt"]
+    target = np.asarray(meta["target"])
+    log(f"C4 FD check: {n} particles, {meta['substeps']} substeps, {len(fx['rows'])} parameters")
+    sim = MpmSimulation(cfg, max_particles=n)
+    sim.set_particles(d["x"], d["v"])
+    sim.write_field("youngs", d["E"])
+    sim.diff_forward(dt, meta["substeps"], meta["checkpoint_every"])
+    r = sim.read_positions().astype(np.float64).mean(axis=0) - target
+    g = sim.diff_backward(np.tile((2.0 / n) * r, (n, 1)).astype(np.float32))
+    sim.close()
+    worst = {"v": 0.0, "E": 0.0}
+    for row in fx["rows"]:
+        gpu = float(g["v"][row["particle"], row["axis"]] if row["param"] == "v" else g["E"][row["particle"]])
+        fd, noise = row["fd"], abs(row["fd"] - row["fd_quarter"])
+        e = abs(gpu - fd) / abs(fd)
+        if noise <= 1e-2 * abs(fd):
+            worst[row["param"]] = max(worst[row["param"]], e)
+        log(f"  {row['param']} p{row['particle']} fd {fd: .6e} gpu {gpu: .6e} rel {e:.1e} "
+            f"(oracle F0=I {abs(row['oracle_adjoint_F0_I'] - fd) / abs(fd):.1e})")
+        assert abs(gpu - fd) <= 3e-3 * abs(fd) + 4 * noise, (row, gpu)
+    log(f"  worst resolved relative error: v {worst['v']:.2e}, E {worst['E']:.2e}")
