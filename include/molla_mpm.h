@@ -117,6 +117,35 @@ int mpm_destroy(mpm_sim* sim);
 const char* mpm_last_error(const mpm_sim* sim);
 int mpm_abi_version(void);
 
+/* ---- sparse grid: a pool of 4^3 tiles (MpmSimulation::new_sparse,
+ *      mpm.rs:848-853; reserve_sparse_tiles :1308; grid_storage_bytes :1280;
+ *      read_active_tile_ids_with :1430; read_grid_storage_mass_momentum
+ *      :2837) ----
+ * One MPM_MODEL_SHADER scene whose grid storage holds `tile_capacity` tiles
+ * (1 .. ceil(grid_size/4)^3).  Each substep gives slots to the tiles within
+ * one cell of the particles' stencils (cells base-1 .. base+3, the
+ * reference's tile marks) and of legacy rigid spheres (radius + 2 cells).
+ * More marked tiles than the capacity latch the overflow fault: the
+ * substep leaves particle state unchanged and completed_gathers stops, as
+ * for any latched fault, until mpm_reset_status.  mpm_read_status reports
+ * the marked tiles as active_tiles.  The deterministic P2G mode and
+ * mpm_read_grid_mass_momentum need the dense grid (mpm.rs:1265-1268,
+ * :2826-2832). */
+int mpm_create_sparse(const mpm_scene_config* cfg, uint32_t tile_capacity, mpm_sim** out);
+/* Grows the pool (a smaller capacity is a no-op); the grid state starts
+ * over on the next step, the particles keep their history. */
+int mpm_reserve_sparse_tiles(mpm_sim* sim, uint32_t tile_capacity);
+/* Bytes of grid storage and tile metadata (particle state excluded). */
+int mpm_grid_storage_bytes(mpm_sim* sim, uint64_t* bytes);
+/* The physical storage of the last P2G: n_cells = 64 x tiles of storage
+ * (the pool's slots in order, or every tile of a dense grid, tile-major);
+ * mass n_cells floats, momentum 3 n_cells. */
+int mpm_read_grid_storage_mass_momentum(mpm_sim* sim, float* mass, float* momentum, uint32_t n_cells);
+/* Tile ids (tx + W (ty + W tz)) of the pool's slots in slot order; *count
+ * receives the number in use.  Fails with MPM_ERR_FAULT while a fault is
+ * latched. */
+int mpm_read_active_tile_ids(mpm_sim* sim, uint32_t* ids, uint32_t capacity, uint32_t* count);
+
 /* ---- particles (set_particles / set_particles_with_velocity, mpm.rs:1451-1519) ----
  * F = I, C = 0, jp = 0, mass/volume from the config; `velocities` may be NULL. */
 int mpm_set_particles(mpm_sim* sim, const float* positions, const float* velocities, uint32_t n);

@@ -91,6 +91,12 @@ struct mpm_sim {
     uint32_t n_chunks = 0;
     float* affine = nullptr;
     float* staging = nullptr;  // cap * 9 floats
+    // sparse tile pool (mpm_create_sparse; pool_cap = 0: dense grid)
+    uint32_t pool_cap = 0;        // tiles of storage
+    uint32_t* tile_slot = nullptr;   // per tile: 1 + slot in the current substep, 0: none
+    uint32_t* tile_mark = nullptr;   // per tile: marked by the scattering pass (cleared by the compaction)
+    uint32_t* slot_tile = nullptr;   // per slot: its tile
+    uint32_t* defer_list = nullptr;  // particles whose global-atomic scatter waits for the slots
     int list_cur = 0;
     bool grid_dirty = false;
     bool sorted = false;
@@ -247,7 +253,8 @@ void release(mpm_sim* s) {
                     s->list[0], s->list[1], s->status, s->slot_of_id, s->keys, s->vals, s->sort_scratch,
                     s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
                     s->src_slot, s->chunk_box, s->part, s->schunks, s->chunk_of_slot, s->cflags,
-                    s->cexcl, s->scan_scratch, s->smark, s->spos, s->sscan};
+                    s->cexcl, s->scan_scratch, s->smark, s->spos, s->sscan, s->tile_slot, s->tile_mark,
+                    s->slot_tile, s->defer_list};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (float* c : s->ckpt) (void)hipFree(c);
@@ -283,6 +290,13 @@ mpm::StepArgs args(mpm_sim* s, float dt, int list_out) {
     a.write_vc = s->write_vc;
     a.has_E = s->has_E;
     a.box_gather = s->boxes_valid ? 1 : 0;
+    // a sparse pool defers global-atomic scatters, which read v and C back
+    if (s->pool_cap) {
+        a.write_vc = 1;
+        a.tile_slot = s->tile_slot;
+        a.tile_mark = s->tile_mark;
+        a.defer_list = s->defer_list;
+    }
     return a;
 }
 
@@ -299,6 +313,12 @@ mpm::GridDev griddev(mpm_sim* s) {
     g.scount = s->status + mpm::ST_SCOUNT;
     g.chunk_of_slot = s->chunk_of_slot;
     g.part_capacity = s->part_cap;
+    if (s->pool_cap) {
+        g.tile_slot = s->tile_slot;
+        g.slot_tile = s->slot_tile;
+        g.pool_count = s->status + mpm::ST_POOL;
+        g.pool_capacity = s->pool_cap;
+    }
     return g;
 }
 
@@ -371,11 +391,22 @@ int particle_blocks(mpm_sim* s) {
     return int(s->n_chunks);
 }
 
+// Cells of grid storage: every tile of a dense grid, the pool of a sparse one.
+size_t grid_cells(const mpm_sim* s) { return (size_t)(s->pool_cap ? s->pool_cap : s->total_tiles) * 64; }
+
 // The dense tile grid is allocated on first use, so contexts that only order
 // particles (or read keys) never pay for it.
 int ensure_grid(mpm_sim* s) {
     if (s->gmp) return MPM_OK;
-    const size_t cells = (size_t)s->total_tiles * 64;
+    const size_t cells = grid_cells(s);
+    if (s->pool_cap) {
+        HIP_TRY(s, dalloc(&s->tile_slot, s->total_tiles));
+        HIP_TRY(s, dalloc(&s->tile_mark, s->total_tiles));
+        HIP_TRY(s, dalloc(&s->slot_tile, s->pool_cap));
+        HIP_TRY(s, dalloc(&s->defer_list, s->cap));
+        HIP_TRY(s, hipMemsetAsync(s->tile_slot, 0, (size_t)s->total_tiles * 4, s->stream));
+        HIP_TRY(s, hipMemsetAsync(s->tile_mark, 0, (size_t)s->total_tiles * 4, s->stream));
+    }
     HIP_TRY(s, dalloc(&s->gmp, cells));
     HIP_TRY(s, dalloc(&s->gv, cells));
     if (s->jacobi_iters > 0 && !s->gv2) HIP_TRY(s, dalloc(&s->gv2, cells));
@@ -567,6 +598,20 @@ int deterministic_p2g(mpm_sim* s, float dt) {
     return MPM_OK;
 }
 
+// Sparse pool, after each scattering pass: the legacy spheres' marks, slots
+// for the marked tiles, then the global-atomic scatters that waited for them.
+int pool_update(mpm_sim* s, float dt, int list_out) {
+    if (!s->pool_cap || s->dry) return MPM_OK;
+    if (s->rigid_count && !s->prescribed) {
+        const mpm::RigidDev rg{s->d_rig, nullptr, s->rigid_count, 0u};
+        HIP_TRY(s, mpm::launch_mark_spheres(s->d_scenes, rg, s->tile_mark, s->pool_cap, s->status, s->stream));
+    }
+    HIP_TRY(s, mpm::launch_pool_compact(s->tile_mark, s->total_tiles, s->tile_slot, s->slot_tile, s->pool_cap,
+                                        s->status, s->stream));
+    HIP_TRY(s, mpm::launch_deferred_p2g(args(s, dt, list_out), int(s->sh[0].model), s->stream));
+    return MPM_OK;
+}
+
 // The substep sequence of one step_n call.
 int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
     int rc;
@@ -593,6 +638,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
     }
     // prologue P2G of the current state
     if ((rc = particle_pass(s, mpm::MODE_P2G, dt, s->list_cur)) != MPM_OK) return rc;
+    if ((rc = pool_update(s, dt, s->list_cur)) != MPM_OK) return rc;
     for (uint32_t k = 0; k < substeps; ++k) {
         const bool last = k + 1 == substeps;
         if ((rc = grid_update(s, dt, !last, last)) != MPM_OK) return rc;
@@ -606,6 +652,7 @@ int run_step_n(mpm_sim* s, float dt, uint32_t substeps) {
             if ((rc = particle_pass(s, mpm::MODE_G2P, dt, 1 - s->list_cur)) != MPM_OK) return rc;
             if (!last && (rc = particle_pass(s, mpm::MODE_P2G, dt, 1 - s->list_cur)) != MPM_OK) return rc;
         }
+        if (!last && (rc = pool_update(s, dt, 1 - s->list_cur)) != MPM_OK) return rc;
         s->since_sort++;
         if (!last) s->list_cur = 1 - s->list_cur;
     }
@@ -1485,6 +1532,8 @@ int mpm_set_p2g_mode(mpm_sim* sim, int mode) {
     if (mode < 0 || mode > 2) return fail(sim, MPM_ERR_INVALID, "unknown p2g mode");
     if (mode == MPM_P2G_DETERMINISTIC && sim->sh[0].model != MPM_MODEL_SHADER)
         return fail(sim, MPM_ERR_INVALID, "deterministic P2G is defined for the shader model");
+    if (mode == MPM_P2G_DETERMINISTIC && sim->pool_cap)  // mpm.rs:1265-1268
+        return fail(sim, MPM_ERR_INVALID, "deterministic P2G needs the dense grid; this handle has a sparse tile pool");
     sim->p2g_mode = mode;
     return MPM_OK;
 }
@@ -1497,8 +1546,7 @@ int mpm_set_time_step(mpm_sim* sim, uint32_t jacobi_iters) {
                 return fail(sim, MPM_ERR_INVALID, "the semi-implicit step is defined for MPM_MODEL_SHADER scenes");
     if (jacobi_iters > 1024) return fail(sim, MPM_ERR_INVALID, "jacobi_iters must be <= 1024");
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
-    if (jacobi_iters > 0 && sim->gmp && !sim->gv2)
-        HIP_TRY(sim, dalloc(&sim->gv2, (size_t)sim->total_tiles * 64));
+    if (jacobi_iters > 0 && sim->gmp && !sim->gv2) HIP_TRY(sim, dalloc(&sim->gv2, grid_cells(sim)));
     sim->jacobi_iters = jacobi_iters;
     for (auto& kv : sim->graphs) (void)hipGraphExecDestroy(kv.second);
     sim->graphs.clear();
@@ -1597,6 +1645,9 @@ int mpm_synchronize(mpm_sim* sim) {
 int mpm_read_grid_mass_momentum(mpm_sim* sim, uint32_t scene, float* mass, float* momentum) {
     if (!sim || !mass || !momentum) return MPM_ERR_INVALID;
     if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    if (sim->pool_cap)  // mpm.rs:2826-2832
+        return fail(sim, MPM_ERR_INVALID,
+                    "sparse tile pool: use mpm_read_grid_storage_mass_momentum for the physical slots");
     const uint32_t G = sim->cfg[scene].grid_size;
     const size_t cells = (size_t)G * G * G;
     {
@@ -1621,7 +1672,7 @@ int mpm_read_status(mpm_sim* sim, mpm_status_info* out) {
     HIP_TRY(sim, hipStreamSynchronize(sim->stream));
     out->fault_bits = w[mpm::ST_FAULT];
     out->completed_gathers = w[mpm::ST_GATHERS];
-    out->active_tiles = w[mpm::ST_COUNT0 + sim->list_cur];
+    out->active_tiles = sim->pool_cap ? w[mpm::ST_POOL] : w[mpm::ST_COUNT0 + sim->list_cur];
     out->sorts = sim->sorts;
     out->escapes = w[mpm::ST_ESCAPES];
     out->fallback_chunks = w[mpm::ST_FALLBACK];
@@ -1639,6 +1690,110 @@ int mpm_reset_status(mpm_sim* sim) {
     HIP_TRY(sim, hipMemcpy(sim->status, z, sizeof(z), hipMemcpyHostToDevice));
     HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_ESCAPES, z, 4, hipMemcpyHostToDevice));
     HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_FALLBACK, z, 4, hipMemcpyHostToDevice));
+    HIP_TRY(sim, hipMemcpy(sim->status + mpm::ST_POOL, z, 8, hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+// ---- sparse tile pool (MpmSimulation::new_sparse / reserve_sparse_tiles,
+//      mpm.rs:848-853,1280-1390) ----------------------------------------------
+
+int mpm_create_sparse(const mpm_scene_config* cfg, uint32_t tile_capacity, mpm_sim** out) {
+    if (!cfg || !out) return MPM_ERR_INVALID;
+    int rc = create(cfg, 1, out);
+    if (rc != MPM_OK) return rc;
+    mpm_sim* s = *out;
+    const uint64_t W = s->sh[0].W;
+    if (tile_capacity == 0 || tile_capacity > W * W * W)
+        return fail(s, MPM_ERR_INVALID, "sparse tile capacity must be in [1, ceil(grid_size / 4)^3]");
+    if (cfg->model != MPM_MODEL_SHADER)
+        return fail(s, MPM_ERR_INVALID, "the sparse tile pool is defined for MPM_MODEL_SHADER scenes");
+    s->pool_cap = tile_capacity;
+    return MPM_OK;
+}
+
+int mpm_reserve_sparse_tiles(mpm_sim* sim, uint32_t tile_capacity) {
+    if (!sim) return MPM_ERR_INVALID;
+    const uint64_t W = sim->sh[0].W;
+    if (!sim->pool_cap || tile_capacity == 0 || tile_capacity > W * W * W)
+        return fail(sim, MPM_ERR_INVALID, "MPM compact tile reservation is invalid");
+    if (tile_capacity <= sim->pool_cap) return MPM_OK;
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    if (sim->gmp) {
+        // new storage; grid state starts over on the next step (particles keep their history)
+        const size_t cells = (size_t)tile_capacity * 64;
+        float4 *gmp = nullptr, *gv = nullptr, *gv2 = nullptr;
+        uint32_t* slot_tile = nullptr;
+        HIP_TRY(sim, dalloc(&gmp, cells));
+        HIP_TRY(sim, dalloc(&gv, cells));
+        if (sim->gv2) HIP_TRY(sim, dalloc(&gv2, cells));
+        HIP_TRY(sim, dalloc(&slot_tile, tile_capacity));
+        HIP_TRY(sim, hipMemsetAsync(gmp, 0, cells * sizeof(float4), sim->stream));
+        HIP_TRY(sim, hipMemsetAsync(gv, 0, cells * sizeof(float4), sim->stream));
+        HIP_TRY(sim, hipMemsetAsync(sim->tile_slot, 0, (size_t)sim->total_tiles * 4, sim->stream));
+        HIP_TRY(sim, hipMemsetAsync(sim->tile_mark, 0, (size_t)sim->total_tiles * 4, sim->stream));
+        HIP_TRY(sim, hipMemsetAsync(sim->tile_flag, 0, (size_t)sim->total_tiles * 4, sim->stream));
+        const uint32_t z[4] = {0u, 0u, 0u, 0u};
+        HIP_TRY(sim, hipMemcpyAsync(sim->status + mpm::ST_COUNT0, z, 8, hipMemcpyHostToDevice, sim->stream));
+        HIP_TRY(sim, hipMemcpyAsync(sim->status + mpm::ST_POOL, z, 8, hipMemcpyHostToDevice, sim->stream));
+        HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+        (void)hipFree(sim->gmp);
+        (void)hipFree(sim->gv);
+        if (sim->gv2) (void)hipFree(sim->gv2);
+        (void)hipFree(sim->slot_tile);
+        sim->gmp = gmp;
+        sim->gv = gv;
+        sim->gv2 = gv2;
+        sim->slot_tile = slot_tile;
+        sim->grid_dirty = false;
+        sim->sorted = false;  // the static list and flags start over with the next sort
+    }
+    sim->pool_cap = tile_capacity;
+    drop_graphs(sim);
+    return MPM_OK;
+}
+
+int mpm_grid_storage_bytes(mpm_sim* sim, uint64_t* bytes) {
+    if (!sim || !bytes) return MPM_ERR_INVALID;
+    const uint64_t cells = grid_cells(sim);
+    uint64_t b = cells * 32u + (sim->jacobi_iters > 0 ? cells * 16u : 0u);  // gmp + gv (+ the sweep buffer)
+    b += sim->pool_cap ? (uint64_t)sim->total_tiles * 8u + (uint64_t)sim->pool_cap * 4u : 0u;  // slot table, marks
+    *bytes = b;
+    return MPM_OK;
+}
+
+int mpm_read_grid_storage_mass_momentum(mpm_sim* sim, float* mass, float* momentum, uint32_t n_cells) {
+    if (!sim || !mass || !momentum) return MPM_ERR_INVALID;
+    const size_t cells = grid_cells(sim);
+    if (n_cells != cells)
+        return fail(sim, MPM_ERR_INVALID, "n_cells must equal the grid storage (" + std::to_string(cells) + " cells)");
+    int rc = ensure_grid(sim);
+    if (rc != MPM_OK) return rc;
+    float* buf = nullptr;
+    HIP_TRY(sim, hipMalloc((void**)&buf, cells * 4 * sizeof(float)));
+    hipError_t e = mpm::launch_storage_to_host_layout(sim->gmp, uint32_t(cells), buf, buf + cells, sim->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(mass, buf, cells * 4, hipMemcpyDeviceToHost, sim->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(momentum, buf + cells, cells * 12, hipMemcpyDeviceToHost, sim->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return hip_fail(sim, e, "read_grid_storage_mass_momentum");
+    return MPM_OK;
+}
+
+int mpm_read_active_tile_ids(mpm_sim* sim, uint32_t* ids, uint32_t capacity, uint32_t* count) {
+    if (!sim || !count) return MPM_ERR_INVALID;
+    if (!sim->pool_cap) return fail(sim, MPM_ERR_INVALID, "tile ids are read from a sparse tile pool");
+    uint32_t w[2] = {0u, 0u};
+    HIP_TRY(sim, hipMemcpyAsync(w, sim->status, 8, hipMemcpyDeviceToHost, sim->stream));
+    uint32_t pool = 0;
+    HIP_TRY(sim, hipMemcpyAsync(&pool, sim->status + mpm::ST_POOL, 4, hipMemcpyDeviceToHost, sim->stream));
+    HIP_TRY(sim, hipStreamSynchronize(sim->stream));
+    if (w[mpm::ST_FAULT]) return fail(sim, MPM_ERR_FAULT, "MPM tile snapshot has a latched grid fault");
+    const uint32_t n = std::min(pool, sim->pool_cap);
+    *count = n;
+    if (!sim->slot_tile || n == 0) return MPM_OK;
+    if (!ids || capacity < n) return fail(sim, MPM_ERR_INVALID, "ids capacity below the active tile count");
+    HIP_TRY(sim, hipMemcpy(ids, sim->slot_tile, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) ids[i] -= sim->sh[0].tile_base;
     return MPM_OK;
 }
 

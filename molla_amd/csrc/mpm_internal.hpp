@@ -13,6 +13,12 @@
 //     `gmp` holds the P2G accumulation (m, px, py, pz) and `gv` the grid
 //     velocity (vx, vy, vz, m) after the grid update.  Only tiles on the
 //     active list are touched in a substep.
+//   * Sparse grid (a tile pool of fixed capacity, mpm_create_sparse): gmp and
+//     gv hold `capacity` tiles; tile_slot[t] is 1 + the pool slot of tile t in
+//     the current substep, 0 when t has none, and cell storage is
+//     (slot << 6) | local.  Slots go to the tiles marked by the substep's
+//     particles (every tile within one cell of a stencil, the reference's
+//     base-1..base+3 rule) and rigid spheres.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,8 +40,12 @@ constexpr uint32_t FAULT_NUMERICAL = 2u;
 
 // Status words in device memory.
 // ST_COUNT0/1: the two dynamic active-tile lists; ST_SCOUNT: the static list.
+// Sparse tile pool: ST_POOL the tiles marked by the last substep (slots in
+// use, or the demand that overflowed the pool), ST_DEFER the particles whose
+// global-atomic scatter waits for the slots.
 enum StatusWord : int {
-    ST_FAULT = 0, ST_GATHERS = 1, ST_COUNT0 = 2, ST_COUNT1 = 3, ST_SCOUNT = 4, ST_ESCAPES = 5, ST_NCHUNKS = 6, ST_FALLBACK = 7, ST_WORDS = 8
+    ST_FAULT = 0, ST_GATHERS = 1, ST_COUNT0 = 2, ST_COUNT1 = 3, ST_SCOUNT = 4, ST_ESCAPES = 5, ST_NCHUNKS = 6,
+    ST_FALLBACK = 7, ST_POOL = 8, ST_DEFER = 9, ST_WORDS = 10
 };
 
 struct SceneDev {
@@ -102,6 +112,13 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 constexpr int LDS_CELLS = 1600;
 
 struct GridDev {
+    // sparse pool (null for a dense grid): 1 + slot per tile (0: none), the
+    // tile of each slot, and the number of slots in use (status[ST_POOL],
+    // capped at pool_capacity)
+    const uint32_t* tile_slot;
+    const uint32_t* slot_tile;
+    const uint32_t* pool_count;
+    uint32_t pool_capacity;
     float4* gmp;          // atomic accumulation / final P2G kept for readback
     float4* gv;           // grid velocity (vx, vy, vz, m)
     uint32_t* tile_flag;  // FLAG_* bits
@@ -143,9 +160,26 @@ struct StepArgs {
                      // the current positions (the previous pass scattered them)
     float* aff_out;  // or null: 9 planes of a.cap floats that receive each scattered
                      // particle's P2G affine A = m C + stress term (the diff tape)
+    // sparse pool (all null for a dense grid)
+    const uint32_t* tile_slot;  // 1 + slot of each tile in the current substep
+    uint32_t* tile_mark;        // tiles the scattering particles mark (nonzero) for the next compaction
+    uint32_t* defer_list;       // slots of particles whose global-atomic scatter waits for the compaction
 };
 
 struct RigidDev;  // rigid bodies on the grid (mpm_rigid.hpp)
+
+// ---- sparse tile pool (mpm_kernels.hip) ----
+// Slots for the marked tiles (marks cleared), in arrival order; more marked
+// tiles than `capacity` latch FAULT_OVERFLOW.  status[ST_POOL] = marked tiles.
+hipError_t launch_pool_compact(uint32_t* tile_mark, uint32_t total_tiles, uint32_t* tile_slot, uint32_t* slot_tile,
+                               uint32_t capacity, uint32_t* status, hipStream_t s);
+// Marks of legacy rigid spheres (mark_rigids, mpm_active_tiles.wgsl:108-124).
+hipError_t launch_mark_spheres(const SceneDev* scenes, const RigidDev& rg, uint32_t* tile_mark, uint32_t capacity,
+                               uint32_t* status, hipStream_t s);
+// The deferred global-atomic P2G of the particles listed in a.defer_list.
+hipError_t launch_deferred_p2g(const StepArgs& a, int model, hipStream_t s);
+// Mass and momentum of the first `cells` storage cells (pool order).
+hipError_t launch_storage_to_host_layout(const float4* gmp, uint32_t cells, float* mass, float* mom, hipStream_t s);
 
 // ---- launchers (mpm_kernels.hip) ----
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,

@@ -61,13 +61,15 @@ __device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float*
 
 // Three waves per SIMD (168 VGPRs) with three blocks per CU (LDS_CELLS):
 // the pass is latency-bound, and this beats the unconstrained 2 waves/SIMD.
-template <int MODE, int MODEL>
+// SPARSE: the grid is a tile pool (one shader-model scene; a.tile_slot set).
+template <int MODE, int MODEL, bool SPARSE>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_particles(StepArgs a) {
     // The differentiable model scatters in int64: its reverse pass divides by
     // grid masses, so edge cells need each contribution kept to f32 precision.
     constexpr bool WIDE = MODEL == MODEL_CPUDIFF;
     __shared__ ScatterLdsT<WIDE> L;
     if (threadIdx.x == 0) L.box[15] = a.status[ST_FAULT] != 0u ? 1 : 0;
+    if (SPARSE && threadIdx.x < 64) L.tmask[threadIdx.x] = 0u;
     __syncthreads();
     if (L.box[15]) return;  // block-uniform: a latched fault stops the pass
     if (MODE != MODE_P2G && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&a.status[ST_GATHERS], 1u);
@@ -89,10 +91,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         // node reads of each particle come from LDS.  Particles outside the
         // box (escaped ones) read global memory.
         int4 gbox = make_int4(0, 0, 0, 0), gdim = make_int4(0, 0, 0, 0);
-        const bool staged = MODE != MODE_P2G && stage_gather_box(a, sc, ci, L, gbox, gdim);
+        const bool staged = MODE != MODE_P2G && stage_gather_box<WIDE, SPARSE>(a, sc, ci, L, gbox, gdim);
         Scatter sp;
         sp.on = false;
         sp.esc = false;
+        sp.slot = slot;
         if (active) {
             float x[3], F[9];
 #pragma unroll
@@ -220,7 +223,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                         gather([&](int i, int j, int k, float4& g) {
                             const int ix = b[0] + i, iy = b[1] + j, iz = b[2] + k;
                             if (ix < 0 || ix >= G || iy < 0 || iy >= G || iz < 0 || iz >= G) return false;
-                            g = a.gv[cell_index(sc, ix, iy, iz)];
+                            uint32_t c;
+                            if (!cell_storage(sc, SPARSE ? a.tile_slot : nullptr, ix, iy, iz, c)) return false;
+                            g = a.gv[c];
                             return true;
                         });
                     }
@@ -410,7 +415,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 if (a.use_lds && stencil_escapes(a, sc, ch, slot, sp.b)) sp.esc = true;
             }
         }
-        if (MODE != MODE_G2P) block_scatter<true, WIDE>(a, sc, sp, ci, L);
+        if (MODE != MODE_G2P) block_scatter<true, WIDE, SPARSE>(a, sc, sp, ci, L);
         else if (staged) __syncthreads();  // the next chunk's staging overwrites the box
     }
 }
@@ -423,15 +428,20 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
     if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);  // XCD-aware chunk walk
     blocks = std::max(blocks, 1);
     dim3 g(blocks), b(BLOCK);
-#define MPM_LAUNCH(M, D) hipLaunchKernelGGL((k_particles<M, D>), g, b, 0, s, a)
-    if (model == MODEL_SHADER) {
-        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_SHADER);
-        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_SHADER);
-        else MPM_LAUNCH(MODE_G2P2G, MODEL_SHADER);
+#define MPM_LAUNCH(M, D, S) hipLaunchKernelGGL((k_particles<M, D, S>), g, b, 0, s, a)
+    if (model == MODEL_SHADER && a.tile_slot) {
+        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_SHADER, true);
+        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_SHADER, true);
+        else MPM_LAUNCH(MODE_G2P2G, MODEL_SHADER, true);
+    } else if (model == MODEL_SHADER) {
+        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_SHADER, false);
+        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_SHADER, false);
+        else MPM_LAUNCH(MODE_G2P2G, MODEL_SHADER, false);
     } else {
-        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_CPUDIFF);
-        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_CPUDIFF);
-        else MPM_LAUNCH(MODE_G2P2G, MODEL_CPUDIFF);
+        if (a.tile_slot) return hipErrorInvalidValue;  // pools hold shader-model scenes
+        if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_CPUDIFF, false);
+        else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_CPUDIFF, false);
+        else MPM_LAUNCH(MODE_G2P2G, MODEL_CPUDIFF, false);
     }
 #undef MPM_LAUNCH
     return hipGetLastError();
@@ -486,14 +496,15 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
                                                      int zero_after, int use_partials, float4* keep, RigidDev rg) {
-    const uint32_t ns = *g.scount, nd = *dcount;
+    const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
-        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+        uint32_t t, st;
+        walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
-        const uint32_t c = (t << 6) | uint32_t(lane);
+        const uint32_t c = (st << 6) | uint32_t(lane);
         const uint32_t flag = g.tile_flag[t];
         float4 q = (flag & FLAG_GMP) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (use_partials) add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
@@ -546,14 +557,15 @@ hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const ui
 __global__ __launch_bounds__(256) void k_rigid_shell(const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
-    const uint32_t ns = *g.scount, nd = *dcount;
+    const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
-        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+        uint32_t t, st;
+        walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
-        const float4 cv = g.gv[(t << 6) | uint32_t(lane)];
+        const float4 cv = g.gv[(st << 6) | uint32_t(lane)];
         const bool live = in_grid(sc, ix, iy, iz) && cv.w > 1.0e-9f && cv.w == cv.w &&
                           !rigid_masked_legacy(rg, sc, ix, iy, iz);
         const float3 node = node_position(sc, ix, iy, iz);
@@ -606,14 +618,15 @@ __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restri
                                                         const float4* __restrict__ src, float4* __restrict__ dst,
                                                         const uint32_t* __restrict__ dlist,
                                                         const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
-    const uint32_t ns = *g.scount, nd = *dcount;
+    const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
-        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+        uint32_t t, st;
+        walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
-        const uint32_t c = (t << 6) | uint32_t(lane);
+        const uint32_t c = (st << 6) | uint32_t(lane);
         const float4 me = src[c];
         const float m = me.w;
         // empty cells and cells stamped by a legacy sphere keep their velocity
@@ -627,8 +640,13 @@ __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restri
         for (int k = 0; k < 6; ++k) {
             const int jx = ix + off[k][0], jy = iy + off[k][1], jz = iz + off[k][2];
             if (!in_grid(sc, jx, jy, jz)) continue;
-            const uint32_t nc = cell_index(sc, jx, jy, jz);
-            if ((g.tile_flag[nc >> 6] & (FLAG_STATIC | FLAG_DYN)) == 0u) continue;
+            uint32_t nc;
+            if (g.tile_slot) {
+                if (!cell_storage(sc, g.tile_slot, jx, jy, jz, nc)) continue;  // no slot: no P2G mass
+            } else {
+                nc = cell_index(sc, jx, jy, jz);
+                if ((g.tile_flag[nc >> 6] & (FLAG_STATIC | FLAG_DYN)) == 0u) continue;
+            }
             const float4 nb = src[nc];
             if (nb.w <= 1.0e-9f || nb.w != nb.w) continue;
             ax += nb.x;
@@ -661,11 +679,12 @@ __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restri
 __global__ __launch_bounds__(256) void k_copy_tiles(GridDev g, const float4* __restrict__ src, float4* __restrict__ dst,
                                                     const uint32_t* __restrict__ dlist,
                                                     const uint32_t* __restrict__ dcount) {
-    const uint32_t ns = *g.scount, nd = *dcount;
+    const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
-        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
-        dst[(t << 6) | uint32_t(lane)] = src[(t << 6) | uint32_t(lane)];
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+        uint32_t t, st;
+        walk.at(g, dlist, i, t, st);
+        dst[(st << 6) | uint32_t(lane)] = src[(st << 6) | uint32_t(lane)];
     }
 }
 
@@ -694,12 +713,13 @@ hipError_t launch_implicit_sweeps(const SceneDev* scenes, const GridDev& g, floa
 // `also` (may be null) is a second per-cell buffer cleared on the same tiles.
 __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount, float4* also) {
-    const uint32_t ns = *g.scount, nd = *dcount;
+    const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
-        const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
-        g.gmp[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (also) also[(t << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+        uint32_t t, st;
+        walk.at(g, dlist, i, t, st);
+        g.gmp[(st << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (also) also[(st << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (lane == 0) g.tile_flag[t] &= FLAG_STATIC;
     }
 }
@@ -1298,6 +1318,160 @@ hipError_t launch_gather_p2g(const StepArgs& a, const uint32_t* tile_scene, cons
                              const float* affine, const uint32_t* slist, const uint32_t* scount,
                              int blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_gather_p2g, dim3(blocks), dim3(64), 0, s, a, tile_scene, tile_lb, affine, slist, scount);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Sparse tile pool (MpmSimulation::new_sparse, mpm.rs:848-853; tile marking
+// mpm_active_tiles.wgsl:79-130).  Every substep the particles that scatter
+// mark the tiles within one cell of their stencils (block_scatter, from its
+// LDS box; deferred particles directly), legacy rigid spheres mark the tiles
+// their shell can reach, and one pass over the tile table gives the marked
+// tiles the pool's slots, in arrival order, and clears the marks.  A demand
+// above the capacity latches FAULT_OVERFLOW: every later kernel of the
+// substep sees the latch and leaves the particles as they were.
+__global__ __launch_bounds__(256) void k_pool_compact(uint32_t* __restrict__ tile_mark, uint32_t total,
+                                                      uint32_t* __restrict__ tile_slot,
+                                                      uint32_t* __restrict__ slot_tile, uint32_t capacity,
+                                                      uint32_t* __restrict__ status) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        uint32_t s1 = 0u;
+        if (tile_mark[t]) {
+            tile_mark[t] = 0u;
+            const uint32_t slot = atomicAdd(&status[ST_POOL], 1u);
+            if (slot < capacity) {
+                s1 = slot + 1u;
+                slot_tile[slot] = t;
+            } else {
+                latch(status, FAULT_OVERFLOW);
+            }
+        }
+        tile_slot[t] = s1;
+    }
+}
+
+hipError_t launch_pool_compact(uint32_t* tile_mark, uint32_t total_tiles, uint32_t* tile_slot, uint32_t* slot_tile,
+                               uint32_t capacity, uint32_t* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, status + ST_POOL, 0u);
+    const unsigned blocks = std::max(1u, std::min<unsigned>((total_tiles + 255) / 256, 16384u));
+    hipLaunchKernelGGL(k_pool_compact, dim3(blocks), dim3(256), 0, s, tile_mark, total_tiles, tile_slot, slot_tile,
+                       capacity, status);
+    return hipGetLastError();
+}
+
+// One block per legacy sphere: the tiles of its cells within radius + 2 cells
+// (mark_rigids, mpm_active_tiles.wgsl:108-124).  A sphere whose tile range
+// alone exceeds the pool latches the overflow without marking.
+__global__ __launch_bounds__(256) void k_mark_spheres(const SceneDev* __restrict__ scenes, RigidDev rg,
+                                                      uint32_t* __restrict__ tile_mark, uint32_t capacity,
+                                                      uint32_t* __restrict__ status) {
+    const float4* b = rg.body + RIGID_ROWS * blockIdx.x;
+    if (b[4].w != 0.0f) return;  // prescribed solids mark nothing
+    const SceneDev& sc = scenes[0];
+    int lo[3], n[3];
+    const float c[3] = {b[0].x, b[0].y, b[0].z};
+    const float extent = fmaxf(b[0].w, 0.0f) * sc.inv_dx + 2.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float ck = (c[k] - sc.origin[k]) * sc.inv_dx;
+        const int l = max(int(floorf(ck - extent)) - sc.window[k], 0);
+        const int h = min(int(ceilf(ck + extent)) - sc.window[k], int(sc.G) - 1);
+        if (l > h) return;
+        lo[k] = l >> 2;
+        n[k] = (h >> 2) - lo[k] + 1;
+    }
+    const uint64_t count = (uint64_t)n[0] * n[1] * n[2];
+    if (count > capacity) {
+        if (threadIdx.x == 0) latch(status, FAULT_OVERFLOW);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+        const uint32_t x = i % n[0], y = (i / n[0]) % n[1], z = i / (n[0] * n[1]);
+        tile_mark[sc.tile_base + uint32_t(lo[0]) + x + sc.W * (uint32_t(lo[1]) + y + sc.W * (uint32_t(lo[2]) + z))] = 1u;
+    }
+}
+
+hipError_t launch_mark_spheres(const SceneDev* scenes, const RigidDev& rg, uint32_t* tile_mark, uint32_t capacity,
+                               uint32_t* status, hipStream_t s) {
+    if (rg.count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mark_spheres, dim3(rg.count), dim3(256), 0, s, scenes, rg, tile_mark, capacity, status);
+    return hipGetLastError();
+}
+
+// The global-atomic P2G of the particles a sparse pool deferred (escaped
+// stencils, chunks that could not reduce in LDS, the global-atomic mode):
+// the P2G pass's particle math (stress from the FP64 SVD of F,
+// mpm.wgsl:532-599) once the compaction has given their tiles slots.  Sparse
+// pools hold one scene of the shader model.
+__global__ __launch_bounds__(256) void k_deferred_p2g(StepArgs a) {
+    if (a.status[ST_FAULT] != 0u) return;
+    const uint32_t n = a.status[ST_DEFER], cap = a.cap;
+    const float* ps = a.ps;
+    const SceneDev sc = a.scenes[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t slot = a.defer_list[i], boff = 4u * slot;
+        float x[3], v[3], C[9], F[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            x[k] = plane_ld(ps, cap, CX + k, boff);
+            v[k] = plane_ld(ps, cap, CV + k, boff);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            C[k] = plane_ld(ps, cap, CC + k, boff);
+            F[k] = plane_ld(ps, cap, CF + k, boff);
+        }
+        const float jp = plane_ld(ps, cap, CJP, boff), m = plane_ld(ps, cap, CMASS, boff),
+                    vol = plane_ld(ps, cap, CVOL, boff);
+        const float* mrow = mat_row(a, sc, __float_as_uint(plane_ld(ps, cap, CMAT, boff)));
+        float mu = sc.mu, lam = sc.lam;
+        if (a.has_E) {
+            const float E = plane_ld(ps, cap, CE, boff);
+            mu = E / (2.0f * (1.0f + sc.poisson));
+            lam = E * sc.poisson / ((1.0f + sc.poisson) * (1.0f - 2.0f * sc.poisson));
+        }
+        Scatter sp;
+        Svd32 d;
+        svd3<MODEL_SHADER>(F, d);
+        affine_from_svd<MODEL_SHADER>(sc, mrow, d, mu, lam, jp, a.dt, vol, m, C, sp.A);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float xp = (x[k] - sc.origin[k]) * sc.inv_dx;
+            const float bf = floorf(xp - 0.5f);
+            sp.b[k] = int(bf) - sc.window[k];
+            sp.fx[k] = xp - bf;
+            sp.mv[k] = m * v[k];
+        }
+        sp.m = m;
+        sp.on = true;
+        sp.esc = true;
+        sp.slot = slot;
+        scatter_global(a, sc, sp);
+    }
+}
+
+hipError_t launch_deferred_p2g(const StepArgs& a, int model, hipStream_t s) {
+    if (model != MODEL_SHADER) return hipErrorInvalidValue;
+    const unsigned blocks = std::max(1u, std::min<unsigned>((a.cap + 255) / 256, 4096u));
+    hipLaunchKernelGGL(k_deferred_p2g, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, a.status + ST_DEFER, 0u);
+    return hipGetLastError();
+}
+
+__global__ void k_storage_layout(const float4* __restrict__ gmp, uint32_t cells, float* __restrict__ mass,
+                                 float* __restrict__ mom) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
+        const float4 q = gmp[i];
+        mass[i] = q.x;
+        mom[3 * (size_t)i] = q.y;
+        mom[3 * (size_t)i + 1] = q.z;
+        mom[3 * (size_t)i + 2] = q.w;
+    }
+}
+
+hipError_t launch_storage_to_host_layout(const float4* gmp, uint32_t cells, float* mass, float* mom, hipStream_t s) {
+    const unsigned blocks = std::max(1u, std::min<unsigned>((cells + 255) / 256, 8192u));
+    hipLaunchKernelGGL(k_storage_layout, dim3(blocks), dim3(256), 0, s, gmp, cells, mass, mom);
     return hipGetLastError();
 }
 

@@ -46,6 +46,43 @@ __device__ __forceinline__ uint32_t cell_index(const SceneDev& sc, int ix, int i
     return (t << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
 }
 
+// Storage of tile t: the tile itself in a dense grid; in a sparse pool its
+// slot of the current substep, or -1 when the pool gave it none (it then
+// holds no P2G mass and no particle reads it).
+__device__ __forceinline__ int storage_tile(const uint32_t* tile_slot, uint32_t t) {
+    return tile_slot ? int(tile_slot[t]) - 1 : int(t);
+}
+
+// Storage index of cell (ix, iy, iz) in `c`; false when its tile has none.
+__device__ __forceinline__ bool cell_storage(const SceneDev& sc, const uint32_t* tile_slot, int ix, int iy, int iz,
+                                             uint32_t& c) {
+    const uint32_t t = sc.tile_base + uint32_t(ix >> 2) + sc.W * (uint32_t(iy >> 2) + sc.W * uint32_t(iz >> 2));
+    const int st = storage_tile(tile_slot, t);
+    if (st < 0) return false;
+    c = (uint32_t(st) << 6) | uint32_t(ix & 3) | (uint32_t(iy & 3) << 2) | (uint32_t(iz & 3) << 4);
+    return true;
+}
+
+// The tiles one particle marks for the sparse pool: those of cells
+// base-1 .. base+3 on each axis, clamped to the grid (mark_particles,
+// mpm_active_tiles.wgsl:90-106).
+__device__ __forceinline__ void mark_stencil_tiles(uint32_t* tile_mark, const SceneDev& sc, const int* b) {
+    const int last = int(sc.G) - 1;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = max(b[k] - 1, 0);
+        hi[k] = min(b[k] + 3, last);
+        if (lo[k] > hi[k]) return;
+        lo[k] >>= 2;
+        hi[k] >>= 2;
+    }
+    for (int z = lo[2]; z <= hi[2]; ++z)
+        for (int y = lo[1]; y <= hi[1]; ++y)
+            for (int x = lo[0]; x <= hi[0]; ++x)
+                tile_mark[sc.tile_base + uint32_t(x) + sc.W * (uint32_t(y) + sc.W * uint32_t(z))] = 1u;
+}
+
 // The CPU-diff floor (mpm_cpu_diff.rs:527-531): node row iy lies strictly
 // below floor_y.  Evaluated in f64 from the scene's f32 values, where the
 // product is exact, so a row whose f32 position would round onto floor_y is
@@ -114,6 +151,7 @@ __device__ __forceinline__ const float* mat_row(const StepArgs& a, const SceneDe
 struct Scatter {
     bool on;   // the particle scatters this pass
     bool esc;  // ... with global atomics (stencil left its sort neighbourhood)
+    uint32_t slot;  // its particle slot (a sparse pool defers its global-atomic scatter)
     int b[3];
     float fx[3];
     float m;
@@ -125,17 +163,18 @@ struct Scatter {
 // fixed point: on gfx950 LDS integer atomics run at about the LDS read rate
 // while LDS f32 atomic adds are ~20x slower, and integer sums are independent
 // of the order in which lanes and waves arrive.
-//   narrow (the shader model's forward P2G): three 64-bit words per cell,
-//     each added with one ds_add_u64 of raw conversion bit patterns (no
-//     per-contribution offset removal; see NarrowWords):
-//       acc[0]  mass as int64, 2^-50 of the block's largest particle mass, so
-//               a cell's mass keeps f32-like relative precision however small
+//   narrow (the shader model's forward P2G): two 64-bit words and one
+//     32-bit word per cell, each added with one LDS atomic of raw conversion
+//     bit patterns (no per-contribution offset removal; see NarrowWords):
+//       acc[0]  mass, 2^-40 of the block's largest particle mass, so a cell's
+//               mass keeps better than f32 relative precision however small
 //               it is (the grid update's m <= 1e-9 test, mpm.wgsl:1019, is
-//               then decided as in f32 arithmetic);
+//               then decided as in f32 arithmetic); its top 16 bits count
+//               the contributions, which removes the conversion offsets;
 //       acc[1]  (px, py) as two int32 halves, 2^-23 of the block's bound on
 //               any single momentum value;
-//       acc[2]  (pz, 1): pz as int32 plus a count of contributions, which
-//               removes the conversion offsets from all three words;
+//       acc[2]  pz as int32 (the low half of the 64-bit row only: a 32-bit
+//               atomic conflicts on half as many LDS banks as a 64-bit one);
 //   wide (int64, four words): each f32 contribution converts exactly (2^-50
 //     of the largest), for sums whose terms span many magnitudes and cancel
 //     (the adjoint's dJ/d grid v, the differentiable model's P2G).
@@ -147,6 +186,9 @@ struct ScatterLdsT {
     int red[WAVES * 6];
     float redf[WAVES * 2];
     int box[16];
+    // sparse pool: the tiles the box's touched cells mark (bit per tile of
+    // the box's tile range, box[8..13]); cleared after each flush
+    uint32_t tmask[64];
     double scale[2];  // power-of-two fixed-point scale of the mass and momentum channels
 };
 using ScatterLds = ScatterLdsT<false>;
@@ -155,20 +197,26 @@ using ScatterLds = ScatterLdsT<false>;
 // scale) converts with one FMA: the f32 bits of w q + 1.5 * 2^23 are
 // MAGIC32 + round(w q).  The mass converts with one f64 FMA: the f64 bits of
 // w m + 1.5 * 2^52 are MAGIC64 + round(w m) for 0 <= w m < 2^51.  A cell's
-// sums of N such terms carry N times each offset; the third word's high half
-// adds 1 per term, so
-//   acc[2] = N (2^32 + MAGIC32) + sum(pz),  |sum(pz)| < 2^30 < (2^32 + MAGIC32) / 2,
-// gives N by rounding, and with it the exact sums of all three words.
+// sums of N such terms carry N times each offset.  MAGIC64 is 0x4338 << 48,
+// and the mass scale keeps every contribution below 2^39, so 256 of them
+// stay below 2^48:
+//   acc[0] = (N 0x4338 mod 2^16) << 48 + sum(m),
+// whose low 48 bits are the mass sum and whose top 16 bits give N
+// (0x4338 = 8 * 0x867, and 0x867 * 0x1357 = 1 mod 2^13, N <= 256 < 2^13).
+// N then removes N MAGIC32 from each int32 momentum sum.
 struct NarrowWords {
-    static constexpr uint32_t MAGIC32 = 0x4B400000u;                 // bits of 1.5 * 2^23
-    static constexpr unsigned long long MAGIC64 = 0x4338000000000000ull;  // bits of 1.5 * 2^52
-    static constexpr double K = 4294967296.0 + 1262485504.0;        // 2^32 + MAGIC32
-    __device__ static __forceinline__ void unpack(unsigned long long wm, unsigned long long wxy,
-                                                  unsigned long long wz, long long& m, int& x, int& y, int& z) {
-        const unsigned long long n = (unsigned long long)rint((double)wz * (1.0 / K));
-        z = (int)(long long)(wz - n * ((1ull << 32) + MAGIC32));
-        m = (long long)(wm - n * MAGIC64);
-        unpack_lo_hi_raw(wxy - n * ((unsigned long long)MAGIC32 * ((1ull << 32) + 1ull)), x, y);
+    static constexpr uint32_t MAGIC32 = 0x4B400000u;  // bits of 1.5 * 2^23
+    static constexpr int MASS_TOP = 40;               // mass scale: largest particle mass -> 2^40
+    // N, the contributions a cell received, from its mass word (0: untouched)
+    __device__ static __forceinline__ uint32_t count(unsigned long long wm) {
+        return ((uint32_t(wm >> 48) >> 3) * 0x1357u) & 0x1fffu;
+    }
+    __device__ static __forceinline__ void unpack(unsigned long long wm, unsigned long long wxy, uint32_t wz,
+                                                  long long& m, int& x, int& y, int& z) {
+        const uint32_t n = count(wm);
+        m = (long long)(wm & ((1ull << 48) - 1ull));
+        z = (int)(wz - n * MAGIC32);
+        unpack_lo_hi_raw(wxy - (unsigned long long)n * ((unsigned long long)MAGIC32 * ((1ull << 32) + 1ull)), x, y);
     }
     // the two int32 halves of a 64-bit sum lo + hi * 2^32 (each |.| < 2^31)
     __device__ static __forceinline__ void unpack_lo_hi_raw(unsigned long long s, int& lo, int& hi) {
@@ -189,6 +237,10 @@ __device__ __forceinline__ unsigned long long raw64(double a, double b) {
 
 __device__ __forceinline__ unsigned long long pair32(uint32_t lo, uint32_t hi) {
     return (unsigned long long)lo | ((unsigned long long)hi << 32);
+}
+
+__device__ __forceinline__ void lds_add_u32(uint32_t* p, uint32_t v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // round(v * s) as a two's-complement int64 for |v * s| < 2^51: adding
@@ -230,14 +282,16 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
                 float wt = w[0][i] * w[1][j] * w[2][k];
                 float d0 = (float(i) - p.fx[0]) * dx, d1 = (float(j) - p.fx[1]) * dx,
                       d2 = (float(k) - p.fx[2]) * dx;
-                uint32_t c = cell_index(sc, ix, iy, iz);
+                const uint32_t t = cell_index(sc, ix, iy, iz) >> 6;
+                uint32_t c;
+                if (!cell_storage(sc, a.tile_slot, ix, iy, iz, c)) continue;  // pool overflow (latched)
                 float* g = reinterpret_cast<float*>(&a.gmp[c]);
                 global_add(g + 0, wt * p.m);
                 global_add(g + 1, wt * (p.mv[0] + p.A[0] * d0 + p.A[1] * d1 + p.A[2] * d2));
                 global_add(g + 2, wt * (p.mv[1] + p.A[3] * d0 + p.A[4] * d1 + p.A[5] * d2));
                 global_add(g + 3, wt * (p.mv[2] + p.A[6] * d0 + p.A[7] * d1 + p.A[8] * d2));
-                mark_tile(a.tile_flag, a.list_out, a.count_out, c >> 6);
-                mark_gmp(a.tile_flag, c >> 6);
+                mark_tile(a.tile_flag, a.list_out, a.count_out, t);
+                mark_gmp(a.tile_flag, t);
             }
 }
 
@@ -248,7 +302,9 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
 // update finds the chunk through the sort).  Escaped particles, and blocks
 // whose box exceeds LDS_CELLS, scatter with global atomics.  MASS = false
 // skips the mass channel (left 0 in the partials) for scatters whose m is 0.
-template <bool MASS = true, bool WIDE = false>
+// SPARSE: the grid is a tile pool (a.tile_mark etc. are set): the chunk marks
+// its tiles, and global-atomic scatters wait for the pool's slots.
+template <bool MASS = true, bool WIDE = false, bool SPARSE = false>
 __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
                                               uint32_t chunk, ScatterLdsT<WIDE>& L) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -301,11 +357,11 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         // A quadratic B-spline weight product is at most 0.75^3 < 0.422, so
         // with narrow momentum values scaled below 2^23 every contribution w q
         // stays below 0.422 * 2^23 < 2^22 (raw32's range) and 256 of them
-        // below 2^30.  Masses (both forms) and wide values: below 2^50, so
-        // every contribution is below 2^49 (raw64's range) and sums of 256
-        // below 2^57.
+        // below 2^30.  Narrow masses: below 2^40, contributions below 2^39,
+        // sums of 256 below 2^47.  Wide values: below 2^50, contributions
+        // below 2^49, sums of 256 below 2^57.
         const int top = WIDE ? 50 : 23;
-        L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, 50 - exp2_ceil(mm)) : 1.0;
+        L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, (WIDE ? 50 : NarrowWords::MASS_TOP) - exp2_ceil(mm)) : 1.0;
         L.scale[1] = mp > 0.f && mp < 1e30f ? ldexp(1.0, top - exp2_ceil(mp)) : 1.0;
         const int bx = h[0] - l[0] + 1, by = h[1] - l[1] + 1, bz = h[2] - l[2] + 1;
         int use = 2;  // nothing reduced in LDS
@@ -318,6 +374,15 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         L.box[4] = by;
         L.box[5] = bz;
         L.box[6] = use;
+        // sparse pool: the tile range of the box grown by one cell (the
+        // range of tiles its touched cells can mark)
+        const int last = int(sc.G) - 1;
+        L.box[8] = max(l[0] - 1, 0) >> 2;
+        L.box[9] = max(l[1] - 1, 0) >> 2;
+        L.box[10] = max(l[2] - 1, 0) >> 2;
+        L.box[11] = (min(h[0] + 1, last) >> 2) - L.box[8] + 1;
+        L.box[12] = (min(h[1] + 1, last) >> 2) - L.box[9] + 1;
+        L.box[13] = (min(h[2] + 1, last) >> 2) - L.box[10] + 1;
         if (use == 0 && a.use_lds) atomicAdd(&a.status[ST_FALLBACK], 1u);
         if (a.use_lds && chunk < a.part_capacity) {
             a.chunk_box[2 * (size_t)chunk] = make_int4(l[0], l[1], l[2], 0);
@@ -338,8 +403,12 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             if (WIDE && MASS) L.acc[0][i] = 0;
             if (!WIDE) L.acc[0][i] = 0;
             L.acc[1][i] = 0;
-            L.acc[2][i] = 0;
-            if constexpr (WIDE) L.acc[3][i] = 0;
+            if constexpr (WIDE) {
+                L.acc[2][i] = 0;
+                L.acc[3][i] = 0;
+            } else {
+                reinterpret_cast<uint32_t*>(&L.acc[2][0])[i] = 0u;
+            }
         }
         __syncthreads();
         if (lds_on) {
@@ -396,7 +465,8 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                                     } else {
                                         lds_add_u64(&L.acc[0][lrow + i], raw64(w0d[i], MASS ? wm : 0.0));
                                         lds_add_u64(&L.acc[1][lrow + i], pair32(raw32(wt, q0), raw32(wt, q1)));
-                                        lds_add_u64(&L.acc[2][lrow + i], pair32(raw32(wt, q2), 1u));
+                                        lds_add_u32(reinterpret_cast<uint32_t*>(&L.acc[2][0]) + lrow + i,
+                                                    raw32(wt, q2));
                                     }
                                 }
                                 if (i < 2) {
@@ -439,19 +509,68 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             // int32 momenta with one rounding to f32, then power-of-two scales
             const double im = 1.0 / sm64;
             const float ip = (float)(1.0 / sp64);
+            constexpr bool marking = SPARSE;
+            const int tb[3] = {L.box[8], L.box[9], L.box[10]}, tw = L.box[11], th = L.box[12];
             for (int i = tid; i < vol; i += BLOCK) {
                 long long sm;
                 int sx, sy, sz;
-                NarrowWords::unpack(L.acc[0][i], L.acc[1][i], L.acc[2][i], sm, sx, sy, sz);
+                NarrowWords::unpack(L.acc[0][i], L.acc[1][i], reinterpret_cast<const uint32_t*>(&L.acc[2][0])[i],
+                                    sm, sx, sy, sz);
                 part[i] = make_float4(MASS ? (float)((double)sm * im) : 0.f, (float)sx * ip, (float)sy * ip,
                                       (float)sz * ip);
+                if (marking && NarrowWords::count(L.acc[0][i]) != 0u) {
+                    // a touched cell (inside some stencil) marks every tile
+                    // within one cell of it: the union over the chunk's
+                    // particles of their base-1..base+3 tiles
+                    const int row = i / bx;
+                    const int c[3] = {x0 + (i - row * bx), y0 + row % by, z0 + row / by};
+                    int lo[3], hi[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        lo[k] = (max(c[k] - 1, 0) >> 2) - tb[k];
+                        hi[k] = (min(c[k] + 1, int(sc.G) - 1) >> 2) - tb[k];
+                    }
+                    for (int z = lo[2]; z <= hi[2]; ++z)
+                        for (int y = lo[1]; y <= hi[1]; ++y)
+                            for (int x = lo[0]; x <= hi[0]; ++x) {
+                                const int bit = x + tw * (y + th * z);
+                                atomicOr(&L.tmask[bit >> 5], 1u << (bit & 31));
+                            }
+                }
             }
         }
         __syncthreads();
+        if (!WIDE && SPARSE && tid < 64) {
+            // the marked tiles of this chunk to the pool's mark table
+            const int tw = L.box[11], th = L.box[12];
+            uint32_t m = L.tmask[tid];
+            L.tmask[tid] = 0u;
+            while (m) {
+                const int bit = tid * 32 + (__ffs(m) - 1);
+                m &= m - 1u;
+                const int x = bit % tw, y = (bit / tw) % th, z = bit / (tw * th);
+                a.tile_mark[sc.tile_base + uint32_t(L.box[8] + x) +
+                            sc.W * (uint32_t(L.box[9] + y) + sc.W * uint32_t(L.box[10] + z))] = 1u;
+            }
+        }
+    }
+    if (SPARSE && p.on && !p.esc && use != 0) {
+        // a stencil with no node inside the grid touches no cell of the box,
+        // but its clamped window may still reach a face tile (:97-106)
+        const int G = int(sc.G);
+        if (p.b[0] + 2 < 0 || p.b[1] + 2 < 0 || p.b[2] + 2 < 0 || p.b[0] >= G || p.b[1] >= G || p.b[2] >= G)
+            mark_stencil_tiles(a.tile_mark, sc, p.b);
     }
     if (p.on && (p.esc || use == 0)) {
         if (p.esc) atomicAdd(&a.status[ST_ESCAPES], 1u);
-        scatter_global(a, sc, p);
+        if (SPARSE) {
+            // sparse pool: the slots of these tiles are assigned after this
+            // pass, so the global-atomic scatter waits for them
+            mark_stencil_tiles(a.tile_mark, sc, p.b);
+            a.defer_list[atomicAdd(&a.status[ST_DEFER], 1u)] = p.slot;
+        } else {
+            scatter_global(a, sc, p);
+        }
     }
 }
 
@@ -490,7 +609,7 @@ __device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDe
 // Block-uniform; every thread calls it.  Returns whether the box is staged
 // (origin in o, extent in d); the caller must not overwrite L.acc before
 // every thread has finished its gather.
-template <bool WIDE>
+template <bool WIDE, bool SPARSE = false>
 __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneDev& sc, uint32_t ci,
                                                  ScatterLdsT<WIDE>& L, int4& o, int4& d) {
     if (!a.box_gather || !a.use_lds || ci >= a.part_capacity) return false;
@@ -503,8 +622,12 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
     // 16 lanes along x, 16 rows (y, z) per pass
     for (int r = int(threadIdx.x >> 4); r < rows; r += BLOCK / 16) {
         const int lz = int((float(r) + 0.5f) * inv_by), ly = r - lz * d.y;
-        for (int lx = int(threadIdx.x & 15); lx < d.x; lx += 16)
-            stage[lx + d.x * r] = a.gv[cell_index(sc, o.x + lx, o.y + ly, o.z + lz)];
+        for (int lx = int(threadIdx.x & 15); lx < d.x; lx += 16) {
+            uint32_t c;
+            stage[lx + d.x * r] = cell_storage(sc, SPARSE ? a.tile_slot : nullptr, o.x + lx, o.y + ly, o.z + lz, c)
+                                      ? a.gv[c]
+                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
     __syncthreads();
     return true;
@@ -606,6 +729,33 @@ __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& s
         }
     }
 }
+
+// The tiles the grid kernels visit, one wave per tile: the static list of
+// the last sort and the dynamic list of a dense grid, or the slots in use of
+// a sparse pool (whose storage tile is then the slot itself).
+struct TileWalk {
+    uint32_t n, ns;
+    __device__ __forceinline__ TileWalk(const GridDev& g, const uint32_t* dcount) {
+        if (g.tile_slot) {
+            n = min(*g.pool_count, g.pool_capacity);
+            ns = 0;
+        } else {
+            ns = *g.scount;
+            n = ns + *dcount;
+        }
+    }
+    // tile id and storage tile of entry i
+    __device__ __forceinline__ void at(const GridDev& g, const uint32_t* dlist, uint32_t i, uint32_t& t,
+                                       uint32_t& st) const {
+        if (g.tile_slot) {
+            t = g.slot_tile[i];
+            st = i;
+        } else {
+            t = i < ns ? g.slist[i] : dlist[i - ns];
+            st = t;
+        }
+    }
+};
 
 // Tile coordinates and this lane's cell of global tile t (one lane per cell).
 __device__ __forceinline__ void tile_cell(const SceneDev& sc, uint32_t t, int lane, int& tx, int& ty, int& tz,

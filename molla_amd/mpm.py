@@ -111,6 +111,11 @@ _SIGS = {
     "mpm_reset_status": (C.c_int, [C.c_void_p]),
     "mpm_sort_keys_debug": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mpm_read_field_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
+    "mpm_create_sparse": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mpm_reserve_sparse_tiles": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "mpm_grid_storage_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "mpm_read_grid_storage_mass_momentum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "mpm_read_active_tile_ids": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "mpm_diff_backward_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p, C.c_void_p]),
     "mpm_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
@@ -181,7 +186,7 @@ def _ptr(a: Optional[np.ndarray]):
 class MpmSimulation:
     """One MPM simulation (or a batch of independent scenes) on the GPU."""
 
-    def __init__(self, cfg, max_particles: Optional[int] = None):
+    def __init__(self, cfg, max_particles: Optional[int] = None, tile_capacity: Optional[int] = None):
         cfgs = list(cfg) if isinstance(cfg, (list, tuple)) else [cfg]
         if max_particles is not None:  # the capacity is cfg.max_particles, as in the reference
             cfgs = [c.copy(max_particles=int(max_particles)) for c in cfgs]
@@ -190,7 +195,13 @@ class MpmSimulation:
         arr = (SceneConfig * len(cfgs))(*[scene_config(c, max_particles) for c in cfgs])
         h = C.c_void_p()
         L = lib()
-        rc = L.mpm_create_batch(arr, len(cfgs), C.byref(h))
+        self.tile_capacity = tile_capacity
+        if tile_capacity is not None:
+            if len(cfgs) != 1:
+                raise MpmError(1, "a sparse tile pool holds one scene")
+            rc = L.mpm_create_sparse(arr, int(tile_capacity), C.byref(h))
+        else:
+            rc = L.mpm_create_batch(arr, len(cfgs), C.byref(h))
         self._h = h
         if rc != 0:
             msg = L.mpm_last_error(h).decode() if h.value else "create failed"
@@ -214,6 +225,54 @@ class MpmSimulation:
     @classmethod
     def batch(cls, cfgs: Sequence[MpmConfig]) -> "MpmSimulation":
         return cls(list(cfgs))
+
+    @classmethod
+    def new_sparse(cls, cfg: MpmConfig, tile_capacity: int) -> "MpmSimulation":
+        """``MpmSimulation::new_sparse`` (mpm.rs:848): grid storage for
+        `tile_capacity` 4^3 tiles."""
+        return cls(cfg, tile_capacity=tile_capacity)
+
+    # -- sparse tile pool --
+    def active_grid_enabled(self) -> bool:
+        return self.tile_capacity is not None
+
+    def reserve_sparse_tiles(self, tile_capacity: int):
+        self._check(lib().mpm_reserve_sparse_tiles(self._h, int(tile_capacity)))
+        self.tile_capacity = max(self.tile_capacity or 0, int(tile_capacity))
+
+    def grid_storage_bytes(self) -> int:
+        b = C.c_uint64(0)
+        self._check(lib().mpm_grid_storage_bytes(self._h, C.byref(b)))
+        return int(b.value)
+
+    def read_grid_storage_mass_momentum(self):
+        """Physical grid storage of the last P2G: (mass (cells,), momentum (cells, 3))."""
+        tiles = self.tile_capacity if self.tile_capacity is not None else sum(
+            ((c.grid_size + 3) // 4) ** 3 for c in self.cfgs)
+        cells = 64 * tiles
+        mass = np.zeros(cells, np.float32)
+        mom = np.zeros((cells, 3), np.float32)
+        self._check(lib().mpm_read_grid_storage_mass_momentum(self._h, _ptr(mass), _ptr(mom), cells))
+        return mass, mom
+
+    def read_active_tile_ids(self) -> np.ndarray:
+        """World tile ids of the pool's slots, in slot order."""
+        n = C.c_uint32(0)
+        ids = np.zeros(max(int(self.tile_capacity or 0), 1), np.uint32)
+        self._check(lib().mpm_read_active_tile_ids(self._h, _ptr(ids), len(ids), C.byref(n)))
+        return ids[: n.value].copy()
+
+    read_active_tile_ids_with = read_active_tile_ids
+
+    def read_active_grid_stats(self) -> dict:
+        """``read_active_grid_stats_with`` (mpm.rs:1397): the status of the
+        last substep; refuses while a fault is latched."""
+        st = self.read_status()
+        if st["faulted"]:
+            raise MpmError(1, f"MPM grid fault after {st['completed_gathers']} gathers; restore or reset required")
+        return st
+
+    read_active_grid_stats_with = read_active_grid_stats
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

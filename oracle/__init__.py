@@ -142,6 +142,44 @@ def base_cells(cfg, x):
     return out
 
 
+def marked_tiles(cfg, x, spheres=()):
+    """The tiles the reference's sparse pool marks for one substep
+    (mpm_active_tiles.wgsl:79-124): for every particle the tiles of cells
+    base-1 .. base+3 on each axis, clamped to the grid, base = floor(xp -
+    0.5) - window with xp = (x - origin) inv_dx in f32 (:90-106); for every
+    legacy sphere (center, radius) the tiles of its cells within radius + 2
+    cells (:108-124).  Returns the sorted tile ids tx + W (ty + W tz)."""
+    G = int(cfg.grid_size)
+    W = (G + 3) // 4
+    o = np.asarray(cfg.grid_origin, np.float32)
+    inv = np.float32(1.0) / np.float32(cfg.cell_size)
+    win = np.asarray(getattr(cfg, "grid_window_cells", (0, 0, 0)), np.int64)
+    x = np.asarray(x, np.float32).reshape(-1, 3)
+    xp = (x - o) * inv
+    base = np.floor(xp - np.float32(0.5)).astype(np.int64) - win
+    lo = np.maximum(base - 1, 0)
+    hi = np.minimum(base + 3, G - 1)
+    ok = np.all(lo <= hi, axis=1)
+    ids = set()
+    for l3, h3 in zip(lo[ok] // 4, hi[ok] // 4):
+        for tz in range(l3[2], h3[2] + 1):
+            for ty in range(l3[1], h3[1] + 1):
+                for tx in range(l3[0], h3[0] + 1):
+                    ids.add(int(tx + W * (ty + W * tz)))
+    for center, radius in spheres:
+        c = (np.asarray(center, np.float32) - o) * inv
+        ext = np.float32(max(float(radius), 0.0)) * inv + np.float32(2.0)
+        l3 = np.maximum(np.floor(c - ext).astype(np.int64) - win, 0)
+        h3 = np.minimum(np.ceil(c + ext).astype(np.int64) - win, G - 1)
+        if np.any(l3 > h3):
+            continue
+        for tz in range(l3[2] // 4, h3[2] // 4 + 1):
+            for ty in range(l3[1] // 4, h3[1] // 4 + 1):
+                for tx in range(l3[0] // 4, h3[0] // 4 + 1):
+                    ids.add(int(tx + W * (ty + W * tz)))
+    return np.array(sorted(ids), np.uint32)
+
+
 # ---------------------------------------------------------------- matrices
 def _m(a):
     return np.ascontiguousarray(a, dtype=np.float64).reshape(9)

@@ -180,17 +180,21 @@ def test_checkpoint_interval_does_not_change_gradients():
     rng = np.random.default_rng(12)
     gx = rng.standard_normal((n, 3)).astype(np.float32)
     gv = np.zeros((n, 3), np.float32)
+    # Bitwise equal when no particle escaped its sort neighbourhood (the
+    # fixed-point partials are order-free); an escaped particle scatters with
+    # f32 global atomics, whose order is free, so then the bound is 2e-5.
     ref = None
     for every in (1, 7, 20):
-        log(f"checkpoint_every={every}")
         g = _gpu_run(cfg, d, dt, steps, every, gx, gv)
+        log(f"checkpoint_every={every}: escapes {g['status']['escapes']}")
         if ref is None:
             ref = g
             continue
+        exact = g["status"]["escapes"] == 0 and ref["status"]["escapes"] == 0
         for k in ("x", "v", "C", "F", "E"):
             e = rel_err(g[k], ref[k])
             log(f"  d/d{k}0 vs every=1: {e:.2e}")
-            assert e < 1e-6, (every, k)
+            assert (e == 0.0) if exact else (e < 2e-5), (every, k, e)
 
 
 def test_centroid_loss_gradient_matches_gpu_finite_difference():

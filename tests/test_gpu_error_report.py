@@ -3,7 +3,9 @@ workloads, as relative errors max|gpu - oracle| / max|oracle| per field.
 
   C1  elastic cube, 8,192 particles, 32^3, 1000 substeps (the whole workload)
   C2  the 2M-particle large scene, cut to 100,000 particles on the same 256^3
-      grid, 20 substeps (bench.py reports the full scene's first substeps)
+      grid, 20 substeps; and at its full density (one elastic block whole and
+      the central column of the fluid pool, ~500,000 particles) over all 200
+      substeps (bench.py reports the full scene's first substeps)
   C3  four scenes of the 512-scene batch (the smallest, the largest and two
       between) stepped as one batch, 100 substeps, each against its own run
   C4  the 65,536-particle diff scene, 64 substeps, checkpoint 16: x_T, and
@@ -91,6 +93,39 @@ def test_c2_large_scene_cut():
     kw = {"v": d["v"], "mass": d["mass"], "vol": d["volume"], "mat": d["material"]}
     o64, o32 = _oracle_pair(cfg, d["x"], meta["dt"], 20, **kw)
     _check("C2 large_scene n=100000 G=256 substeps=20", _errs(g, o64),
+           _errs({f: getattr(o32, f) for f in "xvF"}, o64))
+
+
+def test_c2_full_density_cut_200_substeps():
+    # The large scene at its real density over its whole 200 substeps: one of
+    # the four elastic blocks whole (250,000 particles) and the fluid pool cut
+    # to its central 48 x 48-cell column (about 250,000 particles), on the
+    # same 256^3 grid.  FP64 reference: the oracle's threaded f64 step.
+    cfg, d, meta = workloads.large_scene()
+    x = d["x"]
+    nf = int(np.sum(d["material"] == 0))
+    ne = (len(x) - nf) // 4
+    dx = cfg.cell_size
+    fl = np.arange(nf)[(np.abs(x[:nf, 0] - 128 * dx) < 24 * dx) & (np.abs(x[:nf, 2] - 128 * dx) < 24 * dx)]
+    sel = np.concatenate([fl, nf + np.arange(ne)])
+    sub = {k: d[k][sel] for k in ("x", "v", "mass", "volume", "material")}
+    steps = meta["substeps"]
+    log(f"C2 full-density cut: {len(fl)} fluid + {ne} elastic particles, {steps} substeps")
+    sim = MpmSimulation(cfg, max_particles=len(sel))
+    sim.set_particles(sub["x"], sub["v"])
+    for k in ("mass", "volume", "material"):
+        sim.write_field(k, sub[k])
+    sim.step_n(meta["dt"], steps)
+    g = {"x": sim.read_positions(), "v": sim.read_velocities(), "F": sim.read_field("F").reshape(-1, 3, 3)}
+    st = sim.read_status()
+    sim.close()
+    assert st["fault_bits"] == 0, st
+    kw = {"v": sub["v"], "mass": sub["mass"], "vol": sub["volume"], "mat": sub["material"]}
+    o64 = oracle_state(cfg, sub["x"], **kw)
+    o32 = oracle_state(cfg, sub["x"], dtype=np.float32, **kw)
+    orc.step_threads(cfg, o64, meta["dt"], steps)
+    orc.step_threads(cfg, o32, meta["dt"], steps)
+    _check(f"C2 full-density cut n={len(sel)} G=256 substeps={steps}", _errs(g, o64),
            _errs({f: getattr(o32, f) for f in "xvF"}, o64))
 
 
