@@ -225,6 +225,21 @@ int mpm_reset_status(mpm_sim* sim); /* reset_active_grid, mpm.rs:1388 */
  * scene, padded to `capacity` with 0xffffffff / identity ids. */
 int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_t* keys,
                         uint32_t* ids);
+/* No counterpart in the reference.  mpm_sort_keys_debug for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc.  The same keys and ids, written to
+ * the caller's DEVICE buffers keys_dev / ids_dev (`capacity` u32 each),
+ * ordered on the handle's stream (mpm_synchronize waits for them).  Nothing
+ * is allocated and the handle's sort state is left as it was.  The radix
+ * sort works in a static device scratch array of the library, sized for
+ * MPM_NOALLOC_MAX_PARTICLES particles in the scene (about 18 MB, held by
+ * every GPU the library runs on until the process ends); a larger scene
+ * fails with MPM_ERR_INVALID.  All calls on one GPU share that array, so
+ * calls that overlap on one GPU (handles driven from several host threads
+ * or streams at once) race. */
+#define MPM_NOALLOC_MAX_PARTICLES 2097152u
+int mpm_sort_keys_noalloc(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_t* keys_dev,
+                          uint32_t* ids_dev);
 
 /* ---- profiling (step_n_profiled_with / read_step_profile, mpm_profile.rs:649-666) ---- */
 typedef enum {

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1411,6 +1412,76 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
 
 }  // namespace
 
+// ---- key query without allocation (mpm_sort_keys_noalloc) ------------------
+//
+// The radix sort of mpm_sort_keys_noalloc works in `scratch`, a static device
+// array sized by radix_scratch_words(MPM_NOALLOC_MAX_PARTICLES): ping-pong
+// keys and ids (2 n words), the per-block digit histogram and its scan
+// (2 x 256 x n/4096 words) and the scan's block sums.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code `scratch`
+// names a host-side stand-in; its device address comes from
+// hipGetSymbolAddress, once per device (noalloc_scratch), and only that
+// address is ever handed to a kernel or a copy.
+constexpr size_t NOALLOC_HIST = 256u * ((MPM_NOALLOC_MAX_PARTICLES + 4095u) / 4096u);
+constexpr size_t NOALLOC_WORDS =
+    2u * size_t(MPM_NOALLOC_MAX_PARTICLES) + 2u * NOALLOC_HIST + NOALLOC_HIST / 1024u + 1024u + 256u;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
+namespace {
+
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex noalloc_mu;
+uint32_t* noalloc_ptr[NOALLOC_DEVICES] = {};
+
+// Device address of `scratch` on the handle's device (current when called).
+int noalloc_scratch(mpm_sim* s, uint32_t** out) {
+    if (s->device < 0 || s->device >= NOALLOC_DEVICES)
+        return fail(s, MPM_ERR_INVALID, "mpm_sort_keys_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[s->device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(s, e, "hipGetSymbolAddress(scratch)");
+        noalloc_ptr[s->device] = static_cast<uint32_t*>(p);
+    }
+    *out = noalloc_ptr[s->device];
+    return MPM_OK;
+}
+
+// Scene-local tile key and id of every particle of one scene (the scene's
+// slots are [pbase, pbase + cnt) in any layout), written at the local id so a
+// stable sort over ids 0..cnt-1 gives the reference order.  Same key formula
+// as k_tile_keys (mpm_particle_order.wgsl:13-25), without the tile base.
+__global__ void k_scene_keys(const float* __restrict__ ps, uint32_t cap, const SceneDev* __restrict__ scenes,
+                             uint32_t scene, uint32_t pbase, uint32_t cnt, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ ids) {
+    const SceneDev& sc = scenes[scene];
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+        const uint32_t slot = pbase + j;
+        uint32_t t[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float c = floorf((ps[(size_t)(mpm::CX + k) * cap + slot] - sc.origin[k]) * sc.inv_dx);
+            const int ci = min(max(int(c) - sc.window[k], 0), int(sc.G) - 1);
+            t[k] = uint32_t(ci) >> 2;
+        }
+        const uint32_t id = __float_as_uint(ps[(size_t)mpm::CID * cap + slot]) - pbase;
+        keys[id] = t[0] + sc.W * (t[1] + sc.W * t[2]);
+        ids[id] = id;
+    }
+}
+
+// Padding of the key query beyond the scene's particles (keys 0xffffffff,
+// identity ids), as the ParticleOrder encoding pads to its capacity.
+__global__ void k_pad_keys(uint32_t* __restrict__ keys, uint32_t* __restrict__ ids, uint32_t from, uint32_t to) {
+    for (uint32_t i = from + blockIdx.x * blockDim.x + threadIdx.x; i < to; i += gridDim.x * blockDim.x) {
+        keys[i] = 0xffffffffu;
+        ids[i] = i;
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int mpm_abi_version(void) { return MPM_ABI_VERSION; }
@@ -1820,6 +1891,40 @@ int mpm_sort_keys_debug(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_
     for (uint32_t i = cnt; i < capacity; ++i) {
         keys[i] = 0xffffffffu;
         ids[i] = i;
+    }
+    return MPM_OK;
+}
+
+int mpm_sort_keys_noalloc(mpm_sim* sim, uint32_t scene, uint32_t capacity, uint32_t* keys_dev, uint32_t* ids_dev) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!keys_dev || !ids_dev) return fail(sim, MPM_ERR_INVALID, "mpm_sort_keys_noalloc: null output buffer");
+    if (scene >= sim->n_scenes) return fail(sim, MPM_ERR_INVALID, "scene out of range");
+    const uint32_t cnt = sim->counts[scene];
+    if (capacity < cnt) return fail(sim, MPM_ERR_INVALID, "capacity below the particle count");
+    if (cnt > MPM_NOALLOC_MAX_PARTICLES)
+        return fail(sim, MPM_ERR_INVALID,
+                    "mpm_sort_keys_noalloc: the scene holds " + std::to_string(cnt) +
+                        " particles, above the no-allocation limit of " +
+                        std::to_string(MPM_NOALLOC_MAX_PARTICLES));
+    const uint32_t W = sim->sh[scene].W;
+    const int bits = int(ceil_log2(std::max(W * W * W, 2u)));
+    if (mpm::radix_scratch_words(cnt) > NOALLOC_WORDS)
+        return fail(sim, MPM_ERR_INVALID, "mpm_sort_keys_noalloc: scratch too small for the sort");
+    HIP_TRY(sim, hipSetDevice(sim->device));
+    uint32_t* scr = nullptr;
+    int rc = noalloc_scratch(sim, &scr);
+    if (rc != MPM_OK) return rc;
+    if (cnt > 0) {
+        const unsigned blocks = std::min<uint32_t>((cnt + 255) / 256, 8192u);
+        hipLaunchKernelGGL(k_scene_keys, dim3(blocks), dim3(256), 0, sim->stream, sim->ps[sim->ps_cur], sim->cap,
+                           sim->d_scenes, scene, sim->pbase[scene], cnt, keys_dev, ids_dev);
+        HIP_TRY(sim, hipGetLastError());
+        HIP_TRY(sim, mpm::radix_sort_pairs(keys_dev, ids_dev, cnt, bits, scr, sim->stream));
+    }
+    if (capacity > cnt) {
+        const unsigned blocks = std::min<uint32_t>((capacity - cnt + 255) / 256, 8192u);
+        hipLaunchKernelGGL(k_pad_keys, dim3(blocks), dim3(256), 0, sim->stream, keys_dev, ids_dev, cnt, capacity);
+        HIP_TRY(sim, hipGetLastError());
     }
     return MPM_OK;
 }

@@ -110,6 +110,7 @@ _SIGS = {
     "mpm_read_status": (C.c_int, [C.c_void_p, C.POINTER(StatusInfo)]),
     "mpm_reset_status": (C.c_int, [C.c_void_p]),
     "mpm_sort_keys_debug": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mpm_sort_keys_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mpm_read_field_device": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]),
     "mpm_create_sparse": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "mpm_reserve_sparse_tiles": (C.c_int, [C.c_void_p, C.c_uint32]),
@@ -490,6 +491,13 @@ class MpmSimulation:
         ids = np.zeros(cap, np.uint32)
         self._check(lib().mpm_sort_keys_debug(self._h, scene, cap, _ptr(keys), _ptr(ids)))
         return keys, ids
+
+    def sort_keys_noalloc(self, keys_dev: int, ids_dev: int, capacity: int, scene: int = 0):
+        """The key query into caller-owned device buffers (addresses of
+        `capacity` u32 each), with no allocation; ordered on the handle's
+        stream (``synchronize`` waits for it)."""
+        self._check(lib().mpm_sort_keys_noalloc(self._h, scene, int(capacity), C.c_void_p(keys_dev),
+                                                 C.c_void_p(ids_dev)))
 
     # -- profiling --
     def set_profiling(self, enabled: bool, stages: bool = True, calls: bool = True):
