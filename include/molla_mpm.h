@@ -196,6 +196,33 @@ int mpm_set_write_vc(mpm_sim* sim, int every_substep);
 /* Launch tuning: blocks for the particle pass (0 = one per chunk). */
 int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks);
 
+/* The whole launch configuration (no counterpart in the reference, whose
+ * workgroups are fixed at 64 threads, mpm.rs:314):
+ *   particle_blocks, grid_blocks  as mpm_set_launch;
+ *   sort_every                    as mpm_set_sort_interval;
+ *   particles_per_thread          1 or 2: particles each thread of the fused
+ *                                 pass takes per chunk of the sorted layout
+ *                                 (a chunk holds 256 x this many; shader
+ *                                 model only, MPM_MODEL_CPUDIFF handles run 1);
+ *   segment_tiles                 4 or 8: the longest run of tiles along x
+ *                                 one chunk may span, which bounds the LDS
+ *                                 box of its scatter (8 only where the group
+ *                                 of eight tiles holds two chunks' worth).
+ * The block is 256 threads (4 waves): with the 168-VGPR cap of three waves
+ * per SIMD and three ~49 KB LDS boxes per CU, it is the size that fills a
+ * CU.  Changing particles_per_thread or segment_tiles re-sorts at the next
+ * step.  Results agree to f32 rounding across configurations (the fixed-point
+ * scatter's scale is per chunk). */
+typedef struct {
+    uint32_t particle_blocks;
+    uint32_t grid_blocks;
+    uint32_t sort_every;
+    uint32_t particles_per_thread;
+    uint32_t segment_tiles;
+} mpm_launch_config;
+int mpm_set_launch_config(mpm_sim* sim, const mpm_launch_config* cfg);
+int mpm_get_launch_config(mpm_sim* sim, mpm_launch_config* cfg);
+
 /* Launch autotuner.  No counterpart in the reference, whose dispatch sizes
  * are fixed; it serves the batched mode, where scene mixes differ widely.
  * Times `trial_substeps` substeps (after a 2-substep warm-up) for candidate
@@ -207,6 +234,10 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks)
  * from (stepping resumes from the same state).  Outputs may be NULL. */
 int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,
                  uint32_t* grid_blocks, uint32_t* sort_every);
+/* The same search with particles_per_thread x segment_tiles first (all four
+ * pairs, shader model), then the three above; `out` (may be NULL) receives
+ * the configuration applied.  mpm_autotune runs this search too. */
+int mpm_autotune_config(mpm_sim* sim, float dt, uint32_t trial_substeps, mpm_launch_config* out);
 
 /* ---- stepping (step / step_n, mpm.rs:2232-2272) ---- */
 int mpm_step_n(mpm_sim* sim, float dt, uint32_t substeps);

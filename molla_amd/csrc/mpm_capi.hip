@@ -32,11 +32,13 @@ constexpr uint32_t CHUNK = 256;
 // positions of its sorts (which follow from the interval and the substeps
 // since the last sort).
 struct GraphKey {
-    uint32_t dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks, sort_every, has_E;
+    uint32_t dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks, sort_every, has_E,
+        chunking;
     bool operator<(const GraphKey& o) const {
         return std::tie(dt_bits, n, mode, ps_cur, list_cur, dirty, sort_start, write_vc, pblocks, gblocks, sort_every,
-                        has_E) < std::tie(o.dt_bits, o.n, o.mode, o.ps_cur, o.list_cur, o.dirty, o.sort_start,
-                                          o.write_vc, o.pblocks, o.gblocks, o.sort_every, o.has_E);
+                        has_E, chunking) < std::tie(o.dt_bits, o.n, o.mode, o.ps_cur, o.list_cur, o.dirty,
+                                                    o.sort_start, o.write_vc, o.pblocks, o.gblocks, o.sort_every,
+                                                    o.has_E, o.chunking);
     }
 };
 
@@ -118,6 +120,9 @@ struct mpm_sim {
     int write_vc = 1;
     uint32_t pblocks = 0;
     uint32_t gblocks = 0;
+    uint32_t ppt = 2;         // particles per thread asked for (shader model)
+    uint32_t seg_tiles = 8;   // longest chunk segment, tiles along x
+    uint32_t chunk_ppt = 1;   // particles per thread of the current sorted chunk list
     // profiling
     int profiling = 0;         // MPM_PROFILE_* flags
     std::vector<ProfRec> prof_pending;
@@ -494,15 +499,18 @@ int sort_particles(mpm_sim* s, bool reorder) {
         HIP_TRY(s, mpm::launch_static_rebuild(s->d_scenes, s->tile_scene, s->tile_flag, s->slist,
                                               s->status + mpm::ST_SCOUNT, s->keys, s->n, s->tile_lb,
                                               s->total_tiles, s->smark, s->spos, s->sscan, s->stream));
+        // several particles per thread only for the shader model's fused
+        // layout (the differentiable model and its adjoint take one)
+        s->chunk_ppt = (by_slot && s->sh[0].model == MPM_MODEL_SHADER) ? s->ppt : 1u;
         HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
                                             s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
-                                            s->chunk_of_slot, s->stream));
+                                            s->chunk_of_slot, CHUNK * s->chunk_ppt, s->seg_tiles, s->stream));
         // previous slot of every new slot: the sorted slots spread over cells
         // inside each chunk, or the reference order's ids mapped to slots
         if (by_slot)
             HIP_TRY(s, mpm::launch_chunk_spread(s->d_scenes, s->schunks, s->status + mpm::ST_NCHUNKS,
                                                 s->max_schunks, s->ps[s->ps_cur], s->cap, s->vals, s->src_slot,
-                                                s->keys, s->stream));
+                                                s->keys, CHUNK * s->chunk_ppt, s->stream));
         else
             HIP_TRY(s, mpm::launch_reorder_index(s->vals, s->slot_of_id, s->src_slot, s->n, s->stream));
         HIP_TRY(s, mpm::launch_reorder(s->ps[s->ps_cur], s->ps[other], s->cap, s->src_slot, s->n, s->stream));
@@ -543,7 +551,8 @@ int particle_pass(mpm_sim* s, int mode, float dt, int list_out, float* ps = null
     if (sorted_chunks) use_sorted_chunks(s, a, blocks);
     else a.box_gather = 0;
     s->boxes_valid = boxes_after;
-    HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, 1, blocks, s->stream));
+    HIP_TRY(s, mpm::launch_particle_pass(mode, int(s->sh[0].model), a, sorted_chunks ? int(s->chunk_ppt) : 1, blocks,
+                                         s->stream));
     return MPM_OK;
 }
 
@@ -1302,11 +1311,13 @@ int read_backward_outputs_device(mpm_sim* s, float* gx0, float* gv0, float* gE, 
 }
 
 // ---------------------------------------------------------------------------
-// Launch autotuner: coordinate search over the persistent particle-pass grid,
-// the grid-update grid and the sort interval, timed with HIP events on the
-// context's stream.  The particle state is backed up first and restored
-// afterwards (unsorted, so the next step re-sorts).
-int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* out_gb, uint32_t* out_se) {
+// Launch autotuner: coordinate search over particles per thread x segment
+// length (shader model), the persistent particle-pass grid, the grid-update
+// grid and the sort interval, timed with HIP events on the context's stream.
+// The particle state is backed up first and restored afterwards (unsorted, so
+// the next step re-sorts).
+int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* out_gb, uint32_t* out_se,
+             uint32_t* out_ppt = nullptr, uint32_t* out_seg = nullptr) {
     if (!std::isfinite(dt) || dt < 0.0f) return fail(s, MPM_ERR_INVALID, "dt must be finite and >= 0");
     if (trial == 0) return fail(s, MPM_ERR_INVALID, "trial_substeps must be >= 1");
     if (s->n == 0) return MPM_OK;
@@ -1337,12 +1348,15 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
         return MPM_OK;
     };
     // ms: per substep over `len` timed substeps after a 2-substep warm-up
+    uint32_t try_ppt = s->ppt, try_seg = s->seg_tiles;
     auto run_trial = [&](uint32_t pb, uint32_t gb, uint32_t se, uint32_t len, float& ms) -> int {
         int r = restore();
         if (r != MPM_OK) return r;
         s->pblocks = pb;
         s->gblocks = gb;
         s->sort_every = se;
+        s->ppt = try_ppt;
+        s->seg_tiles = try_seg;
         if ((r = run_step_n(s, dt, 2)) != MPM_OK) return r;
         HIP_TRY(s, hipEventRecord(ea, s->stream));
         if ((r = run_step_n(s, dt, len)) != MPM_OK) return r;
@@ -1353,12 +1367,29 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
         return MPM_OK;
     };
     uint32_t best_pb = s->pblocks, best_gb = s->gblocks, best_se = s->sort_every;
+    uint32_t best_ppt = s->ppt, best_seg = s->seg_tiles;
     if (timing_event(&ea) != hipSuccess || timing_event(&eb) != hipSuccess) rc = fail(s, MPM_ERR_DEVICE, "event");
     float best = 1e30f, ms = 0.f;
     const uint32_t pbs[] = {0u, 1024u, 2048u, 4096u, 8192u, 16384u};
     const uint32_t gbs[] = {512u, 1024u, 2048u, 4096u, 8192u};
     const uint32_t ses[] = {8u, 16u, 32u, 64u, 128u};
     const uint32_t gb0 = best_gb, se0 = best_se ? best_se : 16u;
+    if (s->sh[0].model == MPM_MODEL_SHADER && s->p2g_mode == MPM_P2G_FUSED) {
+        for (uint32_t pp : {1u, 2u})
+            for (uint32_t sg : {4u, 8u}) {
+                if (rc != MPM_OK) break;
+                try_ppt = pp;
+                try_seg = sg;
+                if ((rc = run_trial(best_pb, gb0, se0, trial, ms)) == MPM_OK && ms < best) {
+                    best = ms;
+                    best_ppt = pp;
+                    best_seg = sg;
+                }
+            }
+        try_ppt = best_ppt;
+        try_seg = best_seg;
+    }
+    best = 1e30f;
     for (uint32_t pb : pbs) {
         if (rc != MPM_OK) break;
         if ((rc = run_trial(pb, gb0, se0, trial, ms)) == MPM_OK && ms < best) {
@@ -1391,6 +1422,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     s->pblocks = best_pb;
     s->gblocks = best_gb;
     s->sort_every = best_se;
+    s->ppt = best_ppt;
+    s->seg_tiles = best_seg;
     s->sorts = sorts0;
     s->w_stepped = w_stepped0;
     s->fault_reset = fault_reset0;
@@ -1407,6 +1440,8 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     if (out_pb) *out_pb = best_pb;
     if (out_gb) *out_gb = best_gb;
     if (out_se) *out_se = best_se;
+    if (out_ppt) *out_ppt = best_ppt;
+    if (out_seg) *out_seg = best_seg;
     return MPM_OK;
 }
 
@@ -1643,6 +1678,40 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks_
     return MPM_OK;
 }
 
+int mpm_set_launch_config(mpm_sim* sim, const mpm_launch_config* cfg) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!cfg) return fail(sim, MPM_ERR_INVALID, "cfg must not be NULL");
+    if (cfg->particles_per_thread < 1 || cfg->particles_per_thread > uint32_t(mpm::MAX_PPT))
+        return fail(sim, MPM_ERR_INVALID, "particles_per_thread must be 1 or 2");
+    if (cfg->segment_tiles != 4 && cfg->segment_tiles != 8)
+        return fail(sim, MPM_ERR_INVALID, "segment_tiles must be 4 or 8");
+    sim->pblocks = cfg->particle_blocks;
+    sim->gblocks = cfg->grid_blocks;
+    sim->sort_every = cfg->sort_every;
+    if (cfg->particles_per_thread != sim->ppt || cfg->segment_tiles != sim->seg_tiles) {
+        // a new chunk list at the next step (a taped forward is not reversible after it)
+        sim->ppt = cfg->particles_per_thread;
+        sim->seg_tiles = cfg->segment_tiles;
+        if (sim->sorted) {
+            sim->sorted = false;
+            sim->boxes_valid = false;
+            sim->layout_gen++;
+        }
+    }
+    return MPM_OK;
+}
+
+int mpm_get_launch_config(mpm_sim* sim, mpm_launch_config* cfg) {
+    if (!sim) return MPM_ERR_INVALID;
+    if (!cfg) return fail(sim, MPM_ERR_INVALID, "cfg must not be NULL");
+    cfg->particle_blocks = sim->pblocks;
+    cfg->grid_blocks = sim->gblocks;
+    cfg->sort_every = sim->sort_every;
+    cfg->particles_per_thread = sim->ppt;
+    cfg->segment_tiles = sim->seg_tiles;
+    return MPM_OK;
+}
+
 // validate_prescribed_step (mpm_prescribed_solids.rs:78-89)
 int check_prescribed_step(mpm_sim* s, float dt) {
     if (s->rigid_count && s->prescribed &&
@@ -1673,7 +1742,8 @@ int mpm_step_n_graph(mpm_sim* sim, float dt, uint32_t substeps) {
                             (!sim->sorted || sim->sort_every == 0 || sim->since_sort >= sim->sort_every);
     GraphKey key{dtb, substeps, uint32_t(sim->p2g_mode), uint32_t(sim->ps_cur), uint32_t(sim->list_cur),
                  uint32_t(sim->grid_dirty), uint32_t(sort_start) | (sim->since_sort << 1), uint32_t(sim->write_vc),
-                 sim->pblocks, sim->gblocks, sim->sort_every, uint32_t(sim->has_E)};
+                 sim->pblocks, sim->gblocks, sim->sort_every, uint32_t(sim->has_E),
+                 sim->ppt | (sim->seg_tiles << 4) | (sim->chunk_ppt << 8)};
     auto it = sim->graphs.find(key);
     if (it == sim->graphs.end()) {
         // Lazily allocated tables must exist before capture.
@@ -1990,6 +2060,15 @@ int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* part
                  uint32_t* grid_blocks_, uint32_t* sort_every) {
     if (!sim) return MPM_ERR_INVALID;
     return autotune(sim, dt, trial_substeps, particle_blocks, grid_blocks_, sort_every);
+}
+
+int mpm_autotune_config(mpm_sim* sim, float dt, uint32_t trial_substeps, mpm_launch_config* out) {
+    if (!sim) return MPM_ERR_INVALID;
+    mpm_launch_config c{};
+    const int rc = autotune(sim, dt, trial_substeps, &c.particle_blocks, &c.grid_blocks, &c.sort_every,
+                            &c.particles_per_thread, &c.segment_tiles);
+    if (rc == MPM_OK && out) *out = c;
+    return rc;
 }
 
 int mpm_diff_read_adjoint(mpm_sim* sim, int field, float* out, uint32_t n) {

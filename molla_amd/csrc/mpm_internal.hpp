@@ -86,6 +86,9 @@ __host__ __device__ inline uint32_t chunk_row_tag(uint32_t ty, uint32_t tz) {
 }
 
 constexpr uint32_t CHUNK_PARTICLES = 256;  // one particle per thread of a block
+// Particles per thread of the shader model's sorted-layout passes: a chunk of
+// the sorted layout holds up to CHUNK_PARTICLES * ppt particles.
+constexpr int MAX_PPT = 2;
 
 constexpr int MAT_STRIDE = 8;  // rheology, dp_alpha, theta_c, theta_s, hardening, vdamp, adamp, bulk
 
@@ -217,7 +220,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_out, int by_slot, hipStream_t s);
 hipError_t launch_chunk_spread(const SceneDev* scenes, const Chunk* chunks, const uint32_t* n_chunks,
                                uint32_t max_chunks, const float* ps, uint32_t cap, const uint32_t* src_in,
-                               uint32_t* src_out, uint32_t* keys, hipStream_t s);
+                               uint32_t* src_out, uint32_t* keys, uint32_t chunk_cap, hipStream_t s);
 hipError_t launch_reorder(const float* src, float* dst, uint32_t cap, const uint32_t* src_slot,
                           uint32_t n, hipStream_t s);
 hipError_t launch_index_slots(const float* ps, uint32_t cap, uint32_t* slot_of_id, uint32_t n,
@@ -279,7 +282,7 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
                                const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
-                               uint32_t* chunk_of_slot, hipStream_t s);
+                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, hipStream_t s);
 
 // ---- radix sort (mpm_sort.hip) ----
 size_t radix_scratch_words(uint32_t n);

@@ -59,15 +59,32 @@ __device__ __forceinline__ void affine_from_svd(const SceneDev& sc, const float*
     for (int i = 0; i < 9; ++i) A[i] = m * C[i] + k * S[i];
 }
 
+// LDS for the scatter inputs of a thread's earlier particles of a chunk.
+template <int N>
+struct StashSpace {
+    ScatterStash s[N - 1];
+    __device__ __forceinline__ ScatterStash* ptr() { return s; }
+};
+template <>
+struct StashSpace<1> {
+    __device__ __forceinline__ ScatterStash* ptr() { return nullptr; }
+};
+
 // Three waves per SIMD (168 VGPRs) with three blocks per CU (LDS_CELLS):
 // the pass is latency-bound, and this beats the unconstrained 2 waves/SIMD.
 // SPARSE: the grid is a tile pool (one shader-model scene; a.tile_slot set).
-template <int MODE, int MODEL, bool SPARSE>
+// PPT: particles per thread (shader model, sorted chunks of up to 256 PPT
+// particles): a thread takes positions t, t + 256, ... of the chunk in turn,
+// and one block scatter reduces all of them, so the chunk's box, its partial
+// and the grid update's work are shared by PPT times the particles.
+template <int MODE, int MODEL, bool SPARSE, int PPT>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_particles(StepArgs a) {
     // The differentiable model scatters in int64: its reverse pass divides by
     // grid masses, so edge cells need each contribution kept to f32 precision.
     constexpr bool WIDE = MODEL == MODEL_CPUDIFF;
+    static_assert(PPT == 1 || MODEL == MODEL_SHADER, "several particles per thread: shader model only");
     __shared__ ScatterLdsT<WIDE> L;
+    __shared__ StashSpace<MODE == MODE_G2P ? 1 : PPT> Lst;
     if (threadIdx.x == 0) L.box[15] = a.status[ST_FAULT] != 0u ? 1 : 0;
     if (SPARSE && threadIdx.x < 64) L.tmask[threadIdx.x] = 0u;
     __syncthreads();
@@ -82,9 +99,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         const uint32_t ci = w.ci;
         const Chunk ch = a.chunks[ci];
         const SceneDev sc = a.scenes[ch.scene];
-        const uint32_t slot = ch.start + threadIdx.x;
-        const uint32_t boff = 4u * slot;
-        const bool active = threadIdx.x < ch.count;
         // Gather staging: the chunk's P2G box of the previous pass holds every
         // stencil this gather reads (same particles, same positions), so the
         // block loads that box of grid velocities into LDS once and the 27
@@ -92,10 +106,27 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
         // box (escaped ones) read global memory.
         int4 gbox = make_int4(0, 0, 0, 0), gdim = make_int4(0, 0, 0, 0);
         const bool staged = MODE != MODE_P2G && stage_gather_box<WIDE, SPARSE>(a, sc, ci, L, gbox, gdim);
+        WaveBounds pre;  // the wave's box and bounds over its earlier particles
         Scatter sp;
+#pragma unroll 1
+        for (int pq = 0; pq < PPT; ++pq) {
+        const uint32_t slot = ch.start + threadIdx.x + uint32_t(pq) * BLOCK;
+        const uint32_t boff = 4u * slot;
+        const bool active = threadIdx.x + uint32_t(pq) * BLOCK < ch.count;
+        float xs[3] = {0.f, 0.f, 0.f};  // the scattered position
+        // every field written here, so nothing of the previous particle stays live
         sp.on = false;
         sp.esc = false;
         sp.slot = slot;
+        sp.m = 0.f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            sp.b[k] = 0;
+            sp.fx[k] = 0.f;
+            sp.mv[k] = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sp.A[k] = 0.f;
         if (active) {
             float x[3], F[9];
 #pragma unroll
@@ -401,13 +432,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 }
             }
             if (ok && MODE != MODE_G2P) {
+                stencil_of<MODEL>(sc, x, sp.b, sp.fx);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    float xp = (x[k] - sc.origin[k]) * sc.inv_dx;
-                    float bf = floorf(xp - 0.5f);
-                    sp.b[k] = int(bf) - (MODEL == MODEL_SHADER ? sc.window[k] : 0);
-                    sp.fx[k] = xp - bf;
                     sp.mv[k] = m * v[k];
+                    xs[k] = x[k];
                 }
                 sp.m = m;
                 sp.on = true;
@@ -415,21 +444,39 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 if (a.use_lds && stencil_escapes(a, sc, ch, slot, sp.b)) sp.esc = true;
             }
         }
-        if (MODE != MODE_G2P) block_scatter<true, WIDE, SPARSE>(a, sc, sp, ci, L);
+        // the earlier particles wait in LDS; the last one scatters them all
+        if (MODE != MODE_G2P && PPT > 1 && pq < PPT - 1) {
+            Lst.ptr()[pq].put(sp, xs);
+            wave_bounds_add(sp, sc.dx, pre);
+        }
+        }
+        if (MODE != MODE_G2P) block_scatter<true, WIDE, SPARSE, PPT>(a, sc, sp, ci, L, Lst.ptr(), &pre);
         else if (staged) __syncthreads();  // the next chunk's staging overwrites the box
     }
 }
 
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
                                 hipStream_t s) {
-    (void)ppt;
+    if (ppt < 1 || ppt > MAX_PPT || (ppt > 1 && model != MODEL_SHADER)) return hipErrorInvalidValue;
     if (a.n_chunks == 0) return hipSuccess;
     if (blocks <= 0 || uint32_t(blocks) > a.n_chunks) blocks = int(a.n_chunks);
     if (blocks > int(NUM_XCD)) blocks &= ~int(NUM_XCD - 1);  // XCD-aware chunk walk
     blocks = std::max(blocks, 1);
     dim3 g(blocks), b(BLOCK);
-#define MPM_LAUNCH(M, D, S) hipLaunchKernelGGL((k_particles<M, D, S>), g, b, 0, s, a)
-    if (model == MODEL_SHADER && a.tile_slot) {
+#define MPM_LAUNCH(M, D, S) hipLaunchKernelGGL((k_particles<M, D, S, 1>), g, b, 0, s, a)
+#define MPM_LAUNCH2(M, S) hipLaunchKernelGGL((k_particles<M, MODEL_SHADER, S, 2>), g, b, 0, s, a)
+    if (ppt == 2) {
+        if (mode == MODE_P2G) {
+            if (a.tile_slot) MPM_LAUNCH2(MODE_P2G, true);
+            else MPM_LAUNCH2(MODE_P2G, false);
+        } else if (mode == MODE_G2P) {
+            if (a.tile_slot) MPM_LAUNCH2(MODE_G2P, true);
+            else MPM_LAUNCH2(MODE_G2P, false);
+        } else {
+            if (a.tile_slot) MPM_LAUNCH2(MODE_G2P2G, true);
+            else MPM_LAUNCH2(MODE_G2P2G, false);
+        }
+    } else if (model == MODEL_SHADER && a.tile_slot) {
         if (mode == MODE_P2G) MPM_LAUNCH(MODE_P2G, MODEL_SHADER, true);
         else if (mode == MODE_G2P) MPM_LAUNCH(MODE_G2P, MODEL_SHADER, true);
         else MPM_LAUNCH(MODE_G2P2G, MODEL_SHADER, true);
@@ -444,6 +491,7 @@ hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt,
         else MPM_LAUNCH(MODE_G2P2G, MODEL_CPUDIFF, false);
     }
 #undef MPM_LAUNCH
+#undef MPM_LAUNCH2
     return hipGetLastError();
 }
 
@@ -955,13 +1003,14 @@ __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t x, int mask) {
     return (uint64_t(hi) << 32) | lo;
 }
 
-// Ascending bitonic sort of one value per thread of a 256-thread block:
+// Ascending bitonic sort of one value per thread of an N-thread block:
 // returns the value at sorted position t.  Partners closer than a wave are
-// exchanged by shuffles; the three steps across waves go through v (LDS).
+// exchanged by shuffles; the steps across waves go through v (LDS).
 // Every thread calls it; v may be read by other threads until the first
 // barrier.
-__device__ __forceinline__ uint64_t block_sort256(uint64_t x, uint64_t* v, uint32_t t) {
-    for (uint32_t k = 2; k <= CHUNK_PARTICLES; k <<= 1)
+template <uint32_t N>
+__device__ __forceinline__ uint64_t block_sort(uint64_t x, uint64_t* v, uint32_t t) {
+    for (uint32_t k = 2; k <= N; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             uint64_t other;
             if (j >= 64u) {
@@ -980,21 +1029,27 @@ __device__ __forceinline__ uint64_t block_sort256(uint64_t x, uint64_t* v, uint3
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_chunk_spread(const SceneDev* __restrict__ scenes,
-                                                      const Chunk* __restrict__ chunks,
-                                                      const uint32_t* __restrict__ n_chunks,
-                                                      const float* __restrict__ ps, uint32_t cap,
-                                                      const uint32_t* __restrict__ src_in,
-                                                      uint32_t* __restrict__ src_out,
-                                                      uint32_t* __restrict__ keys) {
-    __shared__ uint64_t order[CHUNK_PARTICLES];
-    __shared__ uint32_t rank[CHUNK_PARTICLES];
-    __shared__ uint32_t tiles[CHUNK_PARTICLES];
+// One thread per particle of a chunk of up to N particles (N = 256 x the
+// particles per thread of the pass): a thread of the pass takes positions t,
+// t + 256, ..., so each of its passes over the block is rank-major too.
+template <uint32_t N>
+__global__ __launch_bounds__(N) void k_chunk_spread(const SceneDev* __restrict__ scenes,
+                                                    const Chunk* __restrict__ chunks,
+                                                    const uint32_t* __restrict__ n_chunks,
+                                                    const float* __restrict__ ps, uint32_t cap,
+                                                    const uint32_t* __restrict__ src_in,
+                                                    uint32_t* __restrict__ src_out,
+                                                    uint32_t* __restrict__ keys) {
+    constexpr uint32_t TB = N == 256 ? 8 : 9;  // bits of a position
+    constexpr uint64_t TMASK = (1ull << TB) - 1ull;
+    __shared__ uint64_t order[N];
+    __shared__ uint32_t rank[N];
+    __shared__ uint32_t tiles[N];
     const uint32_t nch = *n_chunks;
     const uint32_t t = threadIdx.x;
     for (uint32_t ci = blockIdx.x; ci < nch; ci += gridDim.x) {
         const Chunk ch = chunks[ci];
-        const uint32_t cnt = min(ch.count, CHUNK_PARTICLES);
+        const uint32_t cnt = min(ch.count, N);
         const SceneDev& sc = scenes[ch.scene];
         uint32_t src = 0u, tile = 0u;
         uint64_t base = ~0ull;  // 30-bit base cell key; unused threads sort last
@@ -1012,24 +1067,24 @@ __global__ __launch_bounds__(256) void k_chunk_spread(const SceneDev* __restrict
         }
         // 1. (base, slot) ascending: a particle's rank is its distance from
         //    the first entry of its base's run
-        const uint64_t e = block_sort256(base == ~0ull ? ~0ull : (base << 8) | t, order, t);
+        const uint64_t e = block_sort<N>(base == ~0ull ? ~0ull : (base << TB) | t, order, t);
         order[t] = e;
         __syncthreads();
         if (e != ~0ull) {
-            const uint64_t first = e & ~0xffull;
+            const uint64_t first = e & ~TMASK;
             uint32_t lo = 0u, hi = t;  // lower bound of the run in order[0, t]
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (order[mid] < first) lo = mid + 1u;
                 else hi = mid;
             }
-            rank[uint32_t(e & 0xffu)] = t - lo;
+            rank[uint32_t(e & TMASK)] = t - lo;
         }
         __syncthreads();
         // 2. (rank, base, slot) ascending: the new position of every slot
-        const uint64_t f = block_sort256(base == ~0ull ? ~0ull : (uint64_t(rank[t]) << 38) | (base << 8) | t,
-                                         order, t);
-        const uint32_t from = uint32_t(f & 0xffu);
+        const uint64_t f = block_sort<N>(
+            base == ~0ull ? ~0ull : (uint64_t(rank[t]) << (30 + TB)) | (base << TB) | t, order, t);
+        const uint32_t from = uint32_t(f & TMASK);
         // hand the slot's previous slot and tile key to its new position
         // (rank is free: every thread read its own entry before the sort's barriers)
         rank[t] = src;
@@ -1045,11 +1100,17 @@ __global__ __launch_bounds__(256) void k_chunk_spread(const SceneDev* __restrict
 
 hipError_t launch_chunk_spread(const SceneDev* scenes, const Chunk* chunks, const uint32_t* n_chunks,
                                uint32_t max_chunks, const float* ps, uint32_t cap, const uint32_t* src_in,
-                               uint32_t* src_out, uint32_t* keys, hipStream_t s) {
+                               uint32_t* src_out, uint32_t* keys, uint32_t chunk_cap, hipStream_t s) {
     if (max_chunks == 0) return hipSuccess;
     const unsigned blocks = std::min<unsigned>(max_chunks, 8192u);
-    hipLaunchKernelGGL(k_chunk_spread, dim3(blocks), dim3(256), 0, s, scenes, chunks, n_chunks, ps, cap, src_in,
-                       src_out, keys);
+    if (chunk_cap <= CHUNK_PARTICLES)
+        hipLaunchKernelGGL(k_chunk_spread<CHUNK_PARTICLES>, dim3(blocks), dim3(CHUNK_PARTICLES), 0, s, scenes,
+                           chunks, n_chunks, ps, cap, src_in, src_out, keys);
+    else if (chunk_cap <= CHUNK_PARTICLES * MAX_PPT)
+        hipLaunchKernelGGL(k_chunk_spread<CHUNK_PARTICLES * MAX_PPT>, dim3(blocks), dim3(CHUNK_PARTICLES * MAX_PPT),
+                           0, s, scenes, chunks, n_chunks, ps, cap, src_in, src_out, keys);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

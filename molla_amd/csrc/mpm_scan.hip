@@ -1,22 +1,23 @@
 // mpm_scan.hip -- device-wide exclusive scan of u32 and the device-built
 // chunk list of the fused particle pass.
 //
-// Chunks (one block of the particle pass each) are cut from the sorted
-// particle array so that a chunk never leaves its *segment*: consecutive
-// tiles along x in one tile row of one scene.  A group of eight tiles that
-// holds at least DENSE_GROUP particles is one segment; a sparser group is two
-// segments of four.  A segment's last chunk is partly empty, and eight-tile
-// segments halve the idle lanes this costs on dense scenes, where a chunk
-// covers about one tile anyway.  In a sparse row a chunk may spread over its
-// whole segment, and four tiles keep its P2G box (18 x 6 x 6 cells at the
-// sort, plus the motion since) well inside the block's LDS budget.
+// Chunks (one block of the particle pass each, `chunk_cap` = 256 threads x
+// particles per thread) are cut from the sorted particle array so that a
+// chunk never leaves its *segment*: consecutive tiles along x in one tile row
+// of one scene.  With eight-tile segments allowed, a group of eight tiles
+// that holds at least two chunks' worth of particles is one segment; a
+// sparser group is two segments of four.  A segment's last chunk is partly
+// empty, and eight-tile segments halve the idle lanes this costs on dense
+// scenes, where a chunk covers about one tile anyway.  In a sparse row a
+// chunk may spread over its whole segment, and four tiles keep its P2G box
+// (18 x 6 x 6 cells at the sort, plus the motion since) well inside the
+// block's LDS budget.
 #include <algorithm>
 
 #include "mpm_internal.hpp"
 
 namespace mpm {
 
-constexpr uint32_t DENSE_GROUP = 2 * CHUNK_PARTICLES;  // particles that make 8 tiles one segment
 constexpr int SCAN_THREADS = 1024;
 constexpr int SCAN_ITEMS = 4;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -124,13 +125,14 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 // Chunk list of the sorted particle array.
 // The segment of tile `key`: its first tile and its length along x.  lb is
 // the tile lower-bound table of the sorted keys.
-__device__ __forceinline__ void segment_of(const SceneDev& sc, const uint32_t* lb, uint32_t key, uint32_t& first,
-                                           uint32_t& ntiles) {
+// dense_group: particles that make an eight-tile group one segment (~0u: never).
+__device__ __forceinline__ void segment_of(const SceneDev& sc, const uint32_t* lb, uint32_t key, uint32_t dense_group,
+                                           uint32_t& first, uint32_t& ntiles) {
     const uint32_t tl = key - sc.tile_base;
     const uint32_t tx = tl % sc.W;
     const uint32_t g8 = tx & ~7u, n8 = min(8u, sc.W - g8);
     const uint32_t first8 = key - (tx - g8);
-    if (lb[first8 + n8] - lb[first8] >= DENSE_GROUP) {
+    if (lb[first8 + n8] - lb[first8] >= dense_group) {
         first = first8;
         ntiles = n8;
         return;
@@ -142,20 +144,21 @@ __device__ __forceinline__ void segment_of(const SceneDev& sc, const uint32_t* l
 
 __global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
                               const uint32_t* __restrict__ keys, const uint32_t* __restrict__ lb, uint32_t n,
-                              uint32_t* __restrict__ flags) {
+                              uint32_t chunk_cap, uint32_t dense_group, uint32_t* __restrict__ flags) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t key = keys[j];
         const SceneDev& sc = scenes[tile_scene[key]];
         uint32_t first, nt;
-        segment_of(sc, lb, key, first, nt);
+        segment_of(sc, lb, key, dense_group, first, nt);
         const uint32_t seg_begin = lb[first];
-        flags[j] = ((j - seg_begin) % CHUNK_PARTICLES) == 0u ? 1u : 0u;
+        flags[j] = ((j - seg_begin) % chunk_cap) == 0u ? 1u : 0u;
     }
 }
 
 __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ lb, uint32_t n,
-                             const uint32_t* __restrict__ flags, const uint32_t* __restrict__ excl,
+                             uint32_t chunk_cap, uint32_t dense_group, const uint32_t* __restrict__ flags,
+                             const uint32_t* __restrict__ excl,
                              Chunk* __restrict__ chunks, uint32_t* __restrict__ chunk_of_slot) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         const uint32_t c = excl[j] + flags[j] - 1u;  // chunk holding slot j
@@ -165,12 +168,12 @@ __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t
         const uint32_t scn = tile_scene[key];
         const SceneDev& sc = scenes[scn];
         uint32_t first, nt;
-        segment_of(sc, lb, key, first, nt);
+        segment_of(sc, lb, key, dense_group, first, nt);
         const uint32_t seg_end = lb[first + nt];
         Chunk ch;
         ch.scene = scn;
         ch.start = j;
-        ch.count = min((uint32_t)CHUNK_PARTICLES, seg_end - j);
+        ch.count = min(chunk_cap, seg_end - j);
         // the tile row of the chunk's segment (read by stencil_escapes)
         const uint32_t r = (key - sc.tile_base) / sc.W;
         ch.pad = chunk_row_tag(r % sc.W, r / sc.W);
@@ -189,14 +192,18 @@ hipError_t launch_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s) {
 hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
                                const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
-                               uint32_t* chunk_of_slot, hipStream_t s) {
+                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, hipStream_t s) {
+    if (chunk_cap == 0 || chunk_cap > CHUNK_PARTICLES * MAX_PPT || (seg_tiles != 4 && seg_tiles != 8))
+        return hipErrorInvalidValue;
+    const uint32_t dense_group = seg_tiles == 8 ? 2 * chunk_cap : ~0u;
     const unsigned blocks = std::max(1u, std::min<unsigned>((n + 255) / 256, 8192u));
-    hipLaunchKernelGGL(k_chunk_flags, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, flags);
+    hipLaunchKernelGGL(k_chunk_flags, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, chunk_cap,
+                       dense_group, flags);
     uint32_t* total = nullptr;
     hipError_t e = exclusive_scan_u32(flags, excl, n, scan_scratch, &total, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_chunk_emit, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, flags, excl,
-                       chunks, chunk_of_slot);
+    hipLaunchKernelGGL(k_chunk_emit, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, chunk_cap,
+                       dense_group, flags, excl, chunks, chunk_of_slot);
     hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, s, total, n_chunks);
     return hipGetLastError();
 }

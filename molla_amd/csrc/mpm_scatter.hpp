@@ -159,6 +159,107 @@ struct Scatter {
     float A[9];  // m C + 4/dx^2 stress
 };
 
+// Stencil base cell (window-relative for the shader model) and fractional
+// offset of position x.
+template <int MODEL>
+__device__ __forceinline__ void stencil_of(const SceneDev& sc, const float* x, int* b, float* fx) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float xp = (x[k] - sc.origin[k]) * sc.inv_dx;
+        const float bf = floorf(xp - 0.5f);
+        b[k] = int(bf) - (MODEL == MODEL_SHADER ? sc.window[k] : 0);
+        fx[k] = xp - bf;
+    }
+}
+
+// A wave's scatter box and contribution bounds over particles already
+// reduced (wave-uniform values): box cells lo..hi, and bounds on any single
+// mass and momentum contribution (w <= 1 and |d| <= 1.5 dx).
+struct WaveBounds {
+    int lo[3] = {0x3fffffff, 0x3fffffff, 0x3fffffff};
+    int hi[3] = {-0x3fffffff, -0x3fffffff, -0x3fffffff};
+    float bm = 0.f, bp = 0.f;
+};
+
+// Maximum of non-negative floats (+inf included), whose bit patterns order
+// as integers; a NaN lane counts as 0, as fmaxf would drop it.
+__device__ __forceinline__ float wave_maxf(float v) {
+    return __int_as_float(wave_max(v == v ? __float_as_int(v) : 0));
+}
+
+// Merges one particle per lane into w.  Every lane of the wave must be active.
+__device__ __forceinline__ void wave_bounds_add(const Scatter& p, float dx, WaveBounds& w) {
+    int lo[3] = {0x3fffffff, 0x3fffffff, 0x3fffffff}, hi[3] = {-0x3fffffff, -0x3fffffff, -0x3fffffff};
+    float bm = 0.f, bp = 0.f;
+    if (p.on && !p.esc) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = p.b[k];
+            hi[k] = p.b[k] + 2;
+        }
+        bm = fabsf(p.m);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            bp = fmaxf(bp, fabsf(p.mv[r]) + 1.5f * dx * (fabsf(p.A[3 * r]) + fabsf(p.A[3 * r + 1]) +
+                                                         fabsf(p.A[3 * r + 2])));
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        w.lo[k] = min(w.lo[k], wave_min(lo[k]));
+        w.hi[k] = max(w.hi[k], wave_max(hi[k]));
+    }
+    w.bm = fmaxf(w.bm, wave_maxf(bm));
+    w.bp = fmaxf(w.bp, wave_maxf(bp));
+}
+
+// The scatter inputs of a thread's earlier particles of a chunk (passes with
+// more than one particle per thread): position (the stencil is recomputed
+// from it), mass, momentum, affine and the on/esc flags, one word per lane.
+struct ScatterStash {
+    float x[3][BLOCK];
+    float m[BLOCK];
+    float mv[3][BLOCK];
+    float A[9][BLOCK];
+    uint32_t flags[BLOCK];
+    __device__ __forceinline__ void put(const Scatter& p, const float* x_) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            x[k][t] = x_[k];
+            mv[k][t] = p.mv[k];
+        }
+        m[t] = p.m;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) A[k][t] = p.A[k];
+        flags[t] = (p.on ? 1u : 0u) | (p.esc ? 2u : 0u);
+    }
+    // Relaxed atomic loads (plain ds_read): the compiler does not merge them
+    // with a copy of a register-held Scatter on the other side of a branch,
+    // a merge that would move that Scatter into scratch memory.
+    template <class T>
+    __device__ static __forceinline__ T ld(const T* p) {
+        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __device__ __forceinline__ uint32_t flag_bits() const { return ld(&flags[threadIdx.x]); }
+    __device__ __forceinline__ void get(const SceneDev& sc, uint32_t slot, Scatter& p) const {
+        const int t = threadIdx.x;
+        const uint32_t f = ld(&flags[t]);
+        p.on = (f & 1u) != 0u;
+        p.esc = (f & 2u) != 0u;
+        p.slot = slot;
+        float xx[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            xx[k] = ld(&x[k][t]);
+            p.mv[k] = ld(&mv[k][t]);
+        }
+        stencil_of<MODEL_SHADER>(sc, xx, p.b, p.fx);
+        p.m = ld(&m[t]);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) p.A[k] = ld(&A[k][t]);
+    }
+};
+
 // LDS workspace of the block scatter.  The block's box is accumulated in
 // fixed point: on gfx950 LDS integer atomics run at about the LDS read rate
 // while LDS f32 atomic adds are ~20x slower, and integer sums are independent
@@ -173,16 +274,17 @@ struct Scatter {
 //               the contributions, which removes the conversion offsets;
 //       acc[1]  (px, py) as two int32 halves, 2^-23 of the block's bound on
 //               any single momentum value;
-//       acc[2]  pz as int32 (the low half of the 64-bit row only: a 32-bit
-//               atomic conflicts on half as many LDS banks as a 64-bit one);
+//       accz    pz as int32 (a 32-bit atomic conflicts on half as many LDS
+//               banks as a 64-bit one);
 //   wide (int64, four words): each f32 contribution converts exactly (2^-50
 //     of the largest), for sums whose terms span many magnitudes and cancel
 //     (the adjoint's dJ/d grid v, the differentiable model's P2G).
 template <bool WIDE>
 struct ScatterLdsT {
     // before the scatter the same space stages the gather's grid velocities
-    // (LDS_CELLS float4)
-    alignas(16) unsigned long long acc[WIDE ? 4 : 3][LDS_CELLS];
+    // (LDS_CELLS float4 = the first two rows of acc)
+    alignas(16) unsigned long long acc[WIDE ? 4 : 2][LDS_CELLS];
+    uint32_t accz[WIDE ? 1 : LDS_CELLS];
     int red[WAVES * 6];
     float redf[WAVES * 2];
     int box[16];
@@ -198,11 +300,11 @@ using ScatterLds = ScatterLdsT<false>;
 // MAGIC32 + round(w q).  The mass converts with one f64 FMA: the f64 bits of
 // w m + 1.5 * 2^52 are MAGIC64 + round(w m) for 0 <= w m < 2^51.  A cell's
 // sums of N such terms carry N times each offset.  MAGIC64 is 0x4338 << 48,
-// and the mass scale keeps every contribution below 2^39, so 256 of them
-// stay below 2^48:
+// and the mass scale keeps every contribution below 2^39, so the 512 of a
+// chunk of two particles per thread stay below 2^48:
 //   acc[0] = (N 0x4338 mod 2^16) << 48 + sum(m),
 // whose low 48 bits are the mass sum and whose top 16 bits give N
-// (0x4338 = 8 * 0x867, and 0x867 * 0x1357 = 1 mod 2^13, N <= 256 < 2^13).
+// (0x4338 = 8 * 0x867, and 0x867 * 0x1357 = 1 mod 2^13, N <= 512 < 2^13).
 // N then removes N MAGIC32 from each int32 momentum sum.
 struct NarrowWords {
     static constexpr uint32_t MAGIC32 = 0x4B400000u;  // bits of 1.5 * 2^23
@@ -261,12 +363,6 @@ __device__ __forceinline__ int exp2_ceil(float x) {
     return m == 0.5f ? e - 1 : e;
 }
 
-// Maximum of non-negative floats (+inf included), whose bit patterns order
-// as integers; a NaN lane counts as 0, as fmaxf would drop it.
-__device__ __forceinline__ float wave_maxf(float v) {
-    return __int_as_float(wave_max(v == v ? __float_as_int(v) : 0));
-}
-
 // One particle's 27-node P2G straight to global f32 atomics.
 __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev& sc, const Scatter& p) {
     float w[3][3];
@@ -296,7 +392,9 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
 }
 
 // P2G of one block's particles (chunk `chunk`).  Every thread of the block
-// must call this.  Contributions are reduced with LDS int64 atomics over the
+// must call this, with its last particle in `p` and, for PPT > 1 (shader
+// model only), its PPT - 1 earlier ones in `st` (slots p.slot - BLOCK, ...)
+// with their wave bounds already reduced into `pre`.  Contributions are reduced with LDS int64 atomics over the
 // bounding box of the block's stencils; the box is stored as the chunk's
 // partial with plain stores (no global atomics, no registration: the grid
 // update finds the chunk through the sort).  Escaped particles, and blocks
@@ -304,28 +402,50 @@ __device__ __forceinline__ void scatter_global(const StepArgs& a, const SceneDev
 // skips the mass channel (left 0 in the partials) for scatters whose m is 0.
 // SPARSE: the grid is a tile pool (a.tile_mark etc. are set): the chunk marks
 // its tiles, and global-atomic scatters wait for the pool's slots.
-template <bool MASS = true, bool WIDE = false, bool SPARSE = false>
-__device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& p,
-                                              uint32_t chunk, ScatterLdsT<WIDE>& L) {
+template <bool MASS = true, bool WIDE = false, bool SPARSE = false, int PPT = 1>
+__device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev& sc, const Scatter& last,
+                                              uint32_t chunk, ScatterLdsT<WIDE>& L,
+                                              const ScatterStash* st = nullptr, const WaveBounds* pre = nullptr) {
+    static_assert(PPT == 1 || !WIDE, "several particles per thread: shader model only");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float dx = sc.dx;
-    const bool lds_on = p.on && !p.esc;
-    int lo[3], hi[3];
+    // particle q of this thread (q = PPT - 1 is `last`)
+    // (field by field: an aggregate copy here keeps the structs in scratch)
+    auto particle = [&](int q, Scatter& p) __attribute__((always_inline)) {
+        if (PPT == 1 || q == PPT - 1) {
+            p.on = last.on;
+            p.esc = last.esc;
+            p.slot = last.slot;
+            p.m = last.m;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        lo[k] = wave_min(lds_on ? p.b[k] : 0x3fffffff);
-        hi[k] = wave_max(lds_on ? p.b[k] + 2 : -0x3fffffff);
-    }
-    // bounds on any single contribution: w <= 1 and |d| <= 1.5 dx
-    float bm = 0.f, bp = 0.f;
-    if (lds_on) {
-        bm = fabsf(p.m);
+            for (int k = 0; k < 3; ++k) {
+                p.b[k] = last.b[k];
+                p.fx[k] = last.fx[k];
+                p.mv[k] = last.mv[k];
+            }
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-            bp = fmaxf(bp, fabsf(p.mv[r]) + 1.5f * dx * (fabsf(p.A[3 * r]) + fabsf(p.A[3 * r + 1]) + fabsf(p.A[3 * r + 2])));
-    }
-    bm = wave_maxf(bm);
-    bp = wave_maxf(bp);
+            for (int k = 0; k < 9; ++k) p.A[k] = last.A[k];
+        } else {
+            st[q].get(sc, last.slot - uint32_t(PPT - 1 - q) * BLOCK, p);
+        }
+    };
+    // whether particle q scatters through LDS / needs the global path
+    auto flags_of = [&](int q, bool& on, bool& esc) __attribute__((always_inline)) {
+        if (PPT == 1 || q == PPT - 1) {
+            on = last.on;
+            esc = last.esc;
+        } else {
+            const uint32_t f = st[q].flag_bits();
+            on = (f & 1u) != 0u;
+            esc = (f & 2u) != 0u;
+        }
+    };
+    WaveBounds wb;
+    if (PPT > 1 && pre) wb = *pre;
+    wave_bounds_add(last, dx, wb);
+    const int* lo = wb.lo;
+    const int* hi = wb.hi;
+    const float bm = wb.bm, bp = wb.bp;
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -356,10 +476,11 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
         }
         // A quadratic B-spline weight product is at most 0.75^3 < 0.422, so
         // with narrow momentum values scaled below 2^23 every contribution w q
-        // stays below 0.422 * 2^23 < 2^22 (raw32's range) and 256 of them
-        // below 2^30.  Narrow masses: below 2^40, contributions below 2^39,
-        // sums of 256 below 2^47.  Wide values: below 2^50, contributions
-        // below 2^49, sums of 256 below 2^57.
+        // stays below 0.422 * 2^23 < 2^22 (raw32's range) and the 512 of a
+        // two-particle-per-thread chunk below 2^31.  Narrow masses: below
+        // 2^40, contributions below 2^39, sums of 512 below 2^48.  Wide values
+        // (256 per chunk): below 2^50, contributions below 2^49, sums of 256
+        // below 2^57.
         const int top = WIDE ? 50 : 23;
         L.scale[0] = mm > 0.f && mm < 1e30f ? ldexp(1.0, (WIDE ? 50 : NarrowWords::MASS_TOP) - exp2_ceil(mm)) : 1.0;
         L.scale[1] = mp > 0.f && mp < 1e30f ? ldexp(1.0, top - exp2_ceil(mp)) : 1.0;
@@ -407,11 +528,17 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                 L.acc[2][i] = 0;
                 L.acc[3][i] = 0;
             } else {
-                reinterpret_cast<uint32_t*>(&L.acc[2][0])[i] = 0u;
+                L.accz[i] = 0u;
             }
         }
         __syncthreads();
-        if (lds_on) {
+#pragma unroll 1
+        for (int qi = 0; qi < PPT; ++qi) {
+          bool q_on, q_esc;
+          flags_of(qi, q_on, q_esc);
+          if (q_on && !q_esc) {
+            Scatter p;
+            particle(qi, p);
             float w[3][3];
             quad_weights(p.fx[0], w[0]);
             quad_weights(p.fx[1], w[1]);
@@ -465,8 +592,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                                     } else {
                                         lds_add_u64(&L.acc[0][lrow + i], raw64(w0d[i], MASS ? wm : 0.0));
                                         lds_add_u64(&L.acc[1][lrow + i], pair32(raw32(wt, q0), raw32(wt, q1)));
-                                        lds_add_u32(reinterpret_cast<uint32_t*>(&L.acc[2][0]) + lrow + i,
-                                                    raw32(wt, q2));
+                                        lds_add_u32(&L.accz[lrow + i], raw32(wt, q2));
                                     }
                                 }
                                 if (i < 2) {
@@ -493,6 +619,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
                                   p.b[2] + 2 < G;
             if (__ballot(!interior) == 0) nodes(std::false_type{});
             else nodes(std::true_type{});
+          }
         }
         __syncthreads();
         float4* part = a.part + (size_t)chunk * LDS_CELLS;
@@ -514,8 +641,7 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             for (int i = tid; i < vol; i += BLOCK) {
                 long long sm;
                 int sx, sy, sz;
-                NarrowWords::unpack(L.acc[0][i], L.acc[1][i], reinterpret_cast<const uint32_t*>(&L.acc[2][0])[i],
-                                    sm, sx, sy, sz);
+                NarrowWords::unpack(L.acc[0][i], L.acc[1][i], L.accz[i], sm, sx, sy, sz);
                 part[i] = make_float4(MASS ? (float)((double)sm * im) : 0.f, (float)sx * ip, (float)sy * ip,
                                       (float)sz * ip);
                 if (marking && NarrowWords::count(L.acc[0][i]) != 0u) {
@@ -554,22 +680,30 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
             }
         }
     }
-    if (SPARSE && p.on && !p.esc && use != 0) {
-        // a stencil with no node inside the grid touches no cell of the box,
-        // but its clamped window may still reach a face tile (:97-106)
-        const int G = int(sc.G);
-        if (p.b[0] + 2 < 0 || p.b[1] + 2 < 0 || p.b[2] + 2 < 0 || p.b[0] >= G || p.b[1] >= G || p.b[2] >= G)
-            mark_stencil_tiles(a.tile_mark, sc, p.b);
-    }
-    if (p.on && (p.esc || use == 0)) {
-        if (p.esc) atomicAdd(&a.status[ST_ESCAPES], 1u);
-        if (SPARSE) {
-            // sparse pool: the slots of these tiles are assigned after this
-            // pass, so the global-atomic scatter waits for them
-            mark_stencil_tiles(a.tile_mark, sc, p.b);
-            a.defer_list[atomicAdd(&a.status[ST_DEFER], 1u)] = p.slot;
-        } else {
-            scatter_global(a, sc, p);
+#pragma unroll 1
+    for (int qi = 0; qi < PPT; ++qi) {
+        bool q_on, q_esc;
+        flags_of(qi, q_on, q_esc);
+        if (!q_on || (!SPARSE && !q_esc && use != 0)) continue;  // nothing but the LDS scatter
+        Scatter p;
+        particle(qi, p);
+        if (SPARSE && p.on && !p.esc && use != 0) {
+            // a stencil with no node inside the grid touches no cell of the box,
+            // but its clamped window may still reach a face tile (:97-106)
+            const int G = int(sc.G);
+            if (p.b[0] + 2 < 0 || p.b[1] + 2 < 0 || p.b[2] + 2 < 0 || p.b[0] >= G || p.b[1] >= G || p.b[2] >= G)
+                mark_stencil_tiles(a.tile_mark, sc, p.b);
+        }
+        if (p.on && (p.esc || use == 0)) {
+            if (p.esc) atomicAdd(&a.status[ST_ESCAPES], 1u);
+            if (SPARSE) {
+                // sparse pool: the slots of these tiles are assigned after this
+                // pass, so the global-atomic scatter waits for them
+                mark_stencil_tiles(a.tile_mark, sc, p.b);
+                a.defer_list[atomicAdd(&a.status[ST_DEFER], 1u)] = p.slot;
+            } else {
+                scatter_global(a, sc, p);
+            }
         }
     }
 }

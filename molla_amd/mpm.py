@@ -75,6 +75,15 @@ class SceneConfig(C.Structure):
     ]
 
 
+class LaunchConfig(C.Structure):
+    """mpm_launch_config."""
+    _fields_ = [("particle_blocks", C.c_uint32), ("grid_blocks", C.c_uint32), ("sort_every", C.c_uint32),
+                ("particles_per_thread", C.c_uint32), ("segment_tiles", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class StatusInfo(C.Structure):
     _fields_ = [("fault_bits", C.c_uint32), ("completed_gathers", C.c_uint32),
                 ("active_tiles", C.c_uint32), ("sorts", C.c_uint32), ("escapes", C.c_uint32),
@@ -103,6 +112,9 @@ _SIGS = {
     "mpm_set_time_step": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mpm_set_write_vc": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "mpm_set_launch_config": (C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    "mpm_get_launch_config": (C.c_int, [C.c_void_p, C.POINTER(LaunchConfig)]),
+    "mpm_autotune_config": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32, C.POINTER(LaunchConfig)]),
     "mpm_step_n": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32]),
     "mpm_step_n_graph": (C.c_int, [C.c_void_p, C.c_float, C.c_uint32]),
     "mpm_synchronize": (C.c_int, [C.c_void_p]),
@@ -402,6 +414,21 @@ class MpmSimulation:
     def set_launch(self, particle_blocks: int = 0, grid_blocks: int = 0):
         self._check(lib().mpm_set_launch(self._h, int(particle_blocks), int(grid_blocks)))
 
+    def launch_config(self) -> dict:
+        c = LaunchConfig()
+        self._check(lib().mpm_get_launch_config(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def set_launch_config(self, **kw):
+        """Change any of particle_blocks, grid_blocks, sort_every,
+        particles_per_thread (1, 2) and segment_tiles (4, 8)."""
+        cur = self.launch_config()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise TypeError(f"unknown launch settings {sorted(unknown)}")
+        cur.update(kw)
+        self._check(lib().mpm_set_launch_config(self._h, C.byref(LaunchConfig(**{k: int(v) for k, v in cur.items()}))))
+
     # -- stepping --
     def step(self, dt: float):
         self.step_n(dt, 1)
@@ -516,12 +543,12 @@ class MpmSimulation:
         self._check(lib().mpm_reset_profile(self._h))
 
     def autotune(self, dt: float, trial_substeps: int = 10) -> dict:
-        """Pick the fastest launch sizes and sort interval for this scene mix
-        (the particle state is restored afterwards)."""
-        pb, gb, se = (C.c_uint32(), C.c_uint32(), C.c_uint32())
-        self._check(lib().mpm_autotune(self._h, float(dt), int(trial_substeps), C.byref(pb), C.byref(gb),
-                                       C.byref(se)))
-        return {"particle_blocks": pb.value, "grid_blocks": gb.value, "sort_every": se.value}
+        """Pick the fastest launch configuration (particles per thread and
+        segment length, launch sizes, sort interval) for this scene mix; the
+        particle state is restored afterwards."""
+        c = LaunchConfig()
+        self._check(lib().mpm_autotune_config(self._h, float(dt), int(trial_substeps), C.byref(c)))
+        return c.as_dict()
 
     # -- differentiable path --
     def diff_forward(self, dt: float, steps: int, checkpoint_every: int = 16):

@@ -29,12 +29,14 @@ def oracle_state(cfg, x, v=None, dtype=np.float64, mass=None, vol=None, mat=None
 
 
 def run_gpu(cfg, x, v=None, dt=1e-4, substeps=10, mode="fused", fields=None, sort_every=None,
-            graph=False, write_vc=True):
+            graph=False, write_vc=True, launch=None):
     sim = MpmSimulation(cfg, max_particles=max(len(x), 1))
     sim.set_particles(x, v)
     for name, val in (fields or {}).items():
         sim.write_field(name, val)
     sim.set_p2g_mode(mode)
+    if launch:
+        sim.set_launch_config(**launch)
     if sort_every is not None:
         sim.set_sort_interval(sort_every)
     sim.set_write_vc(write_vc)

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle as orc
-from molla_amd import Fluid, Implicit, MpmConfig, MpmSimulation, Sand, Snow, Viscous, workloads
+from molla_amd import Fluid, Implicit, MpmConfig, MpmError, MpmSimulation, Sand, Snow, Viscous, workloads
 from tests.fixtures import fill_box, order_config, points
 from tests.gpu_util import log, oracle_state, rel_err, run_gpu
 
@@ -120,9 +120,9 @@ def test_single_particle_free_fall_is_exact():
         assert abs(pos[0, 0] - 0.013) < 1e-5 and abs(pos[0, 2] - 0.017) < 1e-5
 
 
-def _compare_with_oracle(cfg, x, dt, substeps, v=None, mode="fused", fields=None, tag=""):
+def _compare_with_oracle(cfg, x, dt, substeps, v=None, mode="fused", fields=None, tag="", launch=None):
     log(f"whole-step parity {tag} mode={mode} n={len(x)} substeps={substeps}")
-    _, g = run_gpu(cfg, x, v, dt, substeps, mode=mode, fields=fields)
+    _, g = run_gpu(cfg, x, v, dt, substeps, mode=mode, fields=fields, launch=launch)
     kw = {}
     if fields:
         kw = {k: fields[k] for k in ("mass", "volume", "material", "youngs") if k in fields}
@@ -413,6 +413,54 @@ def test_escaped_particles_fall_back_to_global_atomics():
     assert e <= max(8 * e32, 1e-6)
 
 
+@pytest.mark.parametrize("ppt,seg", [(1, 4), (1, 8), (2, 4), (2, 8)])
+def test_launch_configurations_match_fp64_oracle_and_repeat_bitwise(ppt, seg):
+    # Particles per thread and segment length change the chunks (256 or 512
+    # particles, runs of 4 or 8 tiles) and with them each chunk's fixed-point
+    # scale, so results differ across configurations at f32 rounding only;
+    # one configuration run twice is bitwise identical.  The cube holds 16
+    # particles per cell, so 8-tile segments and full 512-particle chunks
+    # occur, and with 11 sorts over 60 substeps the sorted chunk lists are
+    # rebuilt under the configuration.
+    cfg, x, meta = workloads.elastic_cube(8192, seed=6)
+    launch = {"particles_per_thread": ppt, "segment_tiles": seg, "sort_every": 6}
+    errs = _compare_with_oracle(cfg, x, meta["dt"], 60, tag=f"ppt{ppt} seg{seg}", launch=launch)
+    _assert_errs(errs, {"x": 1e-6, "v": 1e-3, "F": 1e-5, "C": 1e-3})
+    _, a = run_gpu(cfg, x, None, meta["dt"], 60, launch=launch)
+    _, b = run_gpu(cfg, x, None, meta["dt"], 60, launch=launch)
+    assert a["status"]["escapes"] == 0 and a["status"]["sorts"] >= 10
+    for f in ("x", "v", "C", "F"):
+        assert np.array_equal(a[f], b[f]), f
+
+
+def test_launch_configuration_round_trips_and_rejects_bad_values():
+    cfg, x, meta = workloads.elastic_cube(1024, seed=7)
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x, None)
+    sim.set_launch_config(particle_blocks=64, grid_blocks=128, sort_every=9, particles_per_thread=1, segment_tiles=4)
+    assert sim.launch_config() == {"particle_blocks": 64, "grid_blocks": 128, "sort_every": 9,
+                                   "particles_per_thread": 1, "segment_tiles": 4}
+    for bad in ({"particles_per_thread": 0}, {"particles_per_thread": 3}, {"segment_tiles": 6}):
+        with pytest.raises(MpmError):
+            sim.set_launch_config(**bad)
+    assert sim.launch_config()["particles_per_thread"] == 1
+    # a change of chunking between steps re-sorts and keeps stepping exact
+    # against a handle that ran that configuration throughout
+    sim.step_n(meta["dt"], 5)
+    sim.set_launch_config(particles_per_thread=2, segment_tiles=8)
+    sim.step_n(meta["dt"], 5)
+    ref = MpmSimulation(cfg, max_particles=len(x))
+    ref.set_particles(x, None)
+    ref.set_launch_config(particle_blocks=64, grid_blocks=128, sort_every=9, particles_per_thread=1, segment_tiles=4)
+    ref.step_n(meta["dt"], 5)
+    ref.set_launch_config(particles_per_thread=2, segment_tiles=8)
+    ref.step_n(meta["dt"], 5)
+    assert np.array_equal(sim.read_positions(), ref.read_positions())
+    assert sim.read_status()["sorts"] == 2
+    sim.close()
+    ref.close()
+
+
 def test_autotune_restores_state_and_keeps_results():
     cfg, x, meta = workloads.elastic_cube(8192, seed=5)
     dt = meta["dt"]
@@ -421,6 +469,8 @@ def test_autotune_restores_state_and_keeps_results():
     x0, F0 = sim.read_positions(), sim.read_field("F")
     tuned = sim.autotune(dt, 4)
     log(f"autotune -> {tuned}")
+    assert tuned["particles_per_thread"] in (1, 2) and tuned["segment_tiles"] in (4, 8)
+    assert sim.launch_config() == tuned
     assert np.array_equal(sim.read_positions(), x0) and np.array_equal(sim.read_field("F"), F0)
     st = sim.read_status()
     assert st["fault_bits"] == 0 and st["completed_gathers"] == 0 and st["sorts"] == 0
