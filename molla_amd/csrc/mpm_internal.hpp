@@ -107,11 +107,12 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 // tile lower-bound table of the last sort and sums their partials per cell.
 // A particle whose stencil has left that neighbourhood since the sort is
 // scattered with global atomics instead (and counted in ST_ESCAPES).
-// 1600 cells of three 64-bit words (38 KB; the differentiable model's four
-// int64 channels: 51 KB) let three blocks share a CU's 160 KB LDS; a chunk's
-// box (4 or 8 tiles of one tile row plus the stencil
-// reach and the motion since the last sort) stays well inside it, and a
-// larger box falls back to global atomics (counted in ST_FALLBACK).
+// 1600 cells of two 64-bit words and one 32-bit word (32 KB, plus a 17 KB
+// stash with two particles per thread; the differentiable model's four int64
+// channels: 51 KB) let three blocks share a CU's 160 KB LDS; a chunk's box (4
+// or 8 tiles of one tile row plus the stencil reach and the motion since the
+// last sort) stays well inside it, and a larger box falls back to global
+// atomics (counted in ST_FALLBACK).
 constexpr int LDS_CELLS = 1600;
 
 struct GridDev {

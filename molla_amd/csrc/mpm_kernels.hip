@@ -538,11 +538,8 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
 }
 
 // RIGID: bodies are present (the imprint code stays out of the plain kernel).
-// Eight waves per SIMD (64 VGPRs) without bodies: the update is a chain of
-// dependent loads per tile, so more tiles in flight beat deeper load batches
-// (the imprint needs more registers).
 template <bool RIGID>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 : 8))) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
+__global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
