@@ -392,6 +392,13 @@ int grid_blocks(mpm_sim* s) {
     if (s->gblocks) return int(s->gblocks);
     return 2048;
 }
+// Four waves per tile for small scenes, where the active tiles are about as
+// many as the grid-update blocks or fewer (64 particles per block: a dense
+// scene has a few hundred per tile), one otherwise.  Host values only, so the
+// shape, and with it the summation order, is the same every run.
+int grid_split(mpm_sim* s) {
+    return uint64_t(s->n) <= 64ull * uint64_t(grid_blocks(s)) ? 1 : 0;
+}
 int particle_blocks(mpm_sim* s) {
     if (s->pblocks) return int(std::min(s->pblocks, s->n_chunks));
     return int(s->n_chunks);
@@ -573,7 +580,7 @@ int grid_update(mpm_sim* s, float dt, bool zero_after, bool react = false) {
         // Semi-implicit step (mpm.rs:2515-2530): the sweeps need this substep's
         // tile flags, so the grid update keeps them and the clear follows.
         HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count, nullptr, dt, 0, parts,
-                                           grid_blocks(s), s->stream, nullptr, rigid ? &rg : nullptr));
+                                           grid_blocks(s), s->stream, nullptr, rigid ? &rg : nullptr, grid_split(s)));
         HIP_TRY(s, mpm::launch_implicit_sweeps(s->d_scenes, g, s->gv2, s->jacobi_iters, s->list[s->list_cur],
                                                cur_count, dt, grid_blocks(s), s->stream, rigid ? &rg : nullptr));
         if (rg.wrench && !s->prescribed)
@@ -588,7 +595,7 @@ int grid_update(mpm_sim* s, float dt, bool zero_after, bool react = false) {
     HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, g, s->list[s->list_cur], cur_count,
                                        zero_after ? s->status + mpm::ST_COUNT0 + nxt : nullptr, dt,
                                        zero_after ? 1 : 0, parts, grid_blocks(s), s->stream, nullptr,
-                                       rigid ? &rg : nullptr));
+                                       rigid ? &rg : nullptr, grid_split(s)));
     // the shell force reads the updated velocities; the current list and its
     // count stay valid until the next particle pass
     if (rg.wrench && !s->prescribed)
@@ -908,7 +915,7 @@ int record_segment(mpm_sim* s, float dt, uint32_t len) {
         gk.gv = s->gtape + (size_t)2 * k * cells;
         HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], s->status + mpm::ST_COUNT0 + L,
                                            last ? nullptr : s->status + mpm::ST_COUNT0 + (1 - L), dt, last ? 0 : 1,
-                                           1, gb, s->stream, gk.gv + cells));
+                                           1, gb, s->stream, gk.gv + cells, nullptr, grid_split(s)));
         mpm::StepArgs a = args(s, dt, 1 - L);
         a.ps = s->tape + k * st_words;
         a.ps_out = last ? live : a.ps + st_words;
@@ -1062,7 +1069,7 @@ int recompute_segment(mpm_sim* s, uint32_t seg, uint32_t len, bool checked, uint
         mpm::GridDev gk = g;
         gk.gv = s->gtape + (size_t)2 * k * cells;
         HIP_TRY(s, mpm::launch_grid_update(s->d_scenes, gk, s->list[L], count(L), count(1 - L), dt, 1, 1, gb,
-                                           s->stream, gk.gv + cells));
+                                           s->stream, gk.gv + cells, nullptr, grid_split(s)));
         if (k + 1 == len) break;
         mpm::StepArgs a = args(s, dt, 1 - L);
         a.ps = tk;
@@ -1150,7 +1157,7 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
                 mpm::GridDev gk = g;
                 gk.gmp = gv_at(k) + cells;
                 HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 0, gb,
-                                                s->stream));
+                                                s->stream, grid_split(s)));
                 mpm::AdjArgs qk = adj_args(s, dt, R, tape_at(k), blocks);
                 qk.aff = aff_at(k);
                 if (k > 0)
@@ -1174,7 +1181,8 @@ int diff_backward_pass(mpm_sim* s, bool has_gv, bool checked, Stage& span) {
             HIP_TRY(s, mpm::launch_adj_g2p(q, blocks, s->stream));
             mpm::GridDev gk = g;
             gk.gmp = gvk + cells;
-            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 1, gb, s->stream));
+            HIP_TRY(s, mpm::launch_adj_grid(s->d_scenes, gk, s->gadj, s->list[R], count(R), dt, 1, gb, s->stream,
+                                            grid_split(s)));
             HIP_TRY(s, mpm::launch_adj_p2g(q, blocks, s->stream));
             mpm::GridDev gc = g;
             gc.gmp = s->gadj;

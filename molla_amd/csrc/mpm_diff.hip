@@ -465,19 +465,42 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
 // Grid reverse (mpm_adjoint.rs:324-363).  gmp holds the forward's P2G total
 // (m, p) of every listed tile; gadj holds the atomic part of dJ/d(grid v)
 // and receives (dJ/dm, dJ/dp).
+// SPLIT: the block's four waves sum four ranges of each tile's candidate
+// chunks and wave 0 adds them in order (small scenes; as k_grid_update).
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ scenes, GridDev g, float4* gadj,
                                                   const uint32_t* __restrict__ dlist,
                                                   const uint32_t* __restrict__ dcount, float dt, int use_esc) {
+    constexpr int R = 4;
+    __shared__ float4 wsum[SPLIT ? R - 1 : 1][64];
     const uint32_t ns = *g.scount, nd = *dcount;
-    const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < ns + nd; i += gridDim.x * 4) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
+    const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
+    for (uint32_t i = first; i < ns + nd; i += stride) {
         const uint32_t t = i < ns ? g.slist[i] : dlist[i - ns];
         const SceneDev sc = scenes[g.tile_scene[t]];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (t << 6) | uint32_t(lane);
-        float4 q = use_esc ? gadj[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-        add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
+        float4 q = use_esc && (!SPLIT || wave == 0) ? gadj[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? R : 1);
+        if (SPLIT) {
+            if (wave > 0) wsum[wave - 1][lane] = q;
+            __syncthreads();
+            if (wave == 0) {
+#pragma unroll
+                for (int r = 0; r < R - 1; ++r) {
+                    const float4 v = wsum[r][lane];
+                    q.x += v.x;
+                    q.y += v.y;
+                    q.z += v.z;
+                    q.w += v.w;
+                }
+            }
+            __syncthreads();  // wsum is reused by the next tile
+            if (wave != 0) continue;
+        }
         const float4 mp = g.gmp[c];
         const float m = mp.x;
         float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -592,8 +615,13 @@ hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s) {
 }
 
 hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
-                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_grid, dim3(blocks), dim3(256), 0, s, scenes, g, gadj, dlist, dcount, dt, use_esc);
+                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s, int split) {
+    if (split)
+        hipLaunchKernelGGL(k_adj_grid<true>, dim3(blocks), dim3(256), 0, s, scenes, g, gadj, dlist, dcount, dt,
+                           use_esc);
+    else
+        hipLaunchKernelGGL(k_adj_grid<false>, dim3(blocks), dim3(256), 0, s, scenes, g, gadj, dlist, dcount, dt,
+                           use_esc);
     return hipGetLastError();
 }
 

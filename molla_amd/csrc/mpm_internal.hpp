@@ -189,10 +189,12 @@ hipError_t launch_storage_to_host_layout(const float4* gmp, uint32_t cells, floa
 hipError_t launch_particle_pass(int mode, int model, const StepArgs& a, int ppt, int blocks,
                                 hipStream_t s);
 // keep (may be null): per-cell copy of the P2G total (m, p) of the listed tiles.
+// split: four waves per tile instead of one (same results; for scenes with no
+// more active tiles than blocks).
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
                               int zero_after, int use_partials, int blocks, hipStream_t s,
-                              float4* keep = nullptr, const RigidDev* rigid = nullptr);
+                              float4* keep = nullptr, const RigidDev* rigid = nullptr, int split = 0);
 // Shell force of legacy rigid spheres on the active tiles (after the update).
 hipError_t launch_rigid_shell(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, float dt, const RigidDev& rg, int blocks, hipStream_t s);
@@ -265,8 +267,10 @@ hipError_t launch_adj_g2p(const AdjArgs& q, int blocks, hipStream_t s);
 // Grid reverse over the active tiles: dJ/d(grid v) -> (dJ/dm, dJ/dp) in gadj;
 // g.gmp holds the substep's P2G total (m, p).
 // use_esc = 0: gadj holds no escaped-particle part (it is overwritten).
+// split: four waves per tile (small scenes; same choice as the forward's).
 hipError_t launch_adj_grid(const SceneDev* scenes, const GridDev& g, float4* gadj, const uint32_t* dlist,
-                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s);
+                           const uint32_t* dcount, float dt, int use_esc, int blocks, hipStream_t s,
+                           int split = 0);
 // Reverse P2G + stress: gathers gadj; writes adj x, v, C, F of the pre-state
 // and accumulates adj E and the per-chunk (mu, lambda) sums.
 hipError_t launch_adj_p2g(const AdjArgs& q, int blocks, hipStream_t s);

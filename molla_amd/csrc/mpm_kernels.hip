@@ -538,15 +538,27 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
 }
 
 // RIGID: bodies are present (the imprint code stays out of the plain kernel).
-template <bool RIGID>
+// SPLIT = false: one wave per tile sums its candidate chunks' partials in
+// ascending order (scenes with many more active tiles than blocks).
+// SPLIT = true: the block's four waves sum four ranges of the candidate list
+// and wave 0 adds the range sums in order, so the latency of the ordered
+// partial loads is shared four ways (small scenes: a few hundred tiles, each
+// overlapped by tens of chunks).  The host picks the shape from the particle
+// count and the launch size only, so a run repeats bitwise; the two shapes
+// agree to f32 rounding.
+constexpr int GRID_RANGES = 4;
+template <bool RIGID, bool SPLIT>
 __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
                                                      int zero_after, int use_partials, float4* keep, RigidDev rg) {
+    __shared__ float4 wsum[SPLIT ? GRID_RANGES - 1 : 1][64];
     const TileWalk walk(g, dcount);
-    const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
+    const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
+    for (uint32_t i = first; i < walk.n; i += stride) {
         uint32_t t, st;
         walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
@@ -554,31 +566,51 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (st << 6) | uint32_t(lane);
         const uint32_t flag = g.tile_flag[t];
-        float4 q = (flag & FLAG_GMP) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-        if (use_partials) add_partials(g, sc, tx, ty, tz, ix, iy, iz, q);
-        float4 out;
-        int owner = -1;
-        if (RIGID && rg.count) {
-            // rigid_imprint (mpm.wgsl:937-1006) between the P2G and the update:
-            // a stamped cell keeps the body's velocity (grid_update :1034-1038)
-            float3 force, torque;
-            owner = in_grid(sc, ix, iy, iz) ? rigid_imprint(rg, sc, ix, iy, iz, dt, q, force, torque) : -1;
-            if (rg.wrench) rigid_wrench_add(rg, owner, force, torque);
+        // range 0 starts from the atomic channel
+        float4 q = (flag & FLAG_GMP) && (!SPLIT || wave == 0) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (use_partials) {
+            add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? GRID_RANGES : 1);
+            if (SPLIT) {
+                if (wave > 0) wsum[wave - 1][lane] = q;
+                __syncthreads();
+                if (wave == 0) {
+#pragma unroll
+                    for (int r = 0; r < GRID_RANGES - 1; ++r) {
+                        const float4 v = wsum[r][lane];
+                        q.x += v.x;
+                        q.y += v.y;
+                        q.z += v.z;
+                        q.w += v.w;
+                    }
+                }
+            }
         }
-        if (owner >= 0) {
-            out = make_float4(q.y / q.x, q.z / q.x, q.w / q.x, q.x);
-        } else {
-            cell_update(sc, q, ix, iy, iz, dt, out);
+        if (!SPLIT || wave == 0) {
+            float4 out;
+            int owner = -1;
+            if (RIGID && rg.count) {
+                // rigid_imprint (mpm.wgsl:937-1006) between the P2G and the update:
+                // a stamped cell keeps the body's velocity (grid_update :1034-1038)
+                float3 force, torque;
+                owner = in_grid(sc, ix, iy, iz) ? rigid_imprint(rg, sc, ix, iy, iz, dt, q, force, torque) : -1;
+                if (rg.wrench) rigid_wrench_add(rg, owner, force, torque);
+            }
+            if (owner >= 0) {
+                out = make_float4(q.y / q.x, q.z / q.x, q.w / q.x, q.x);
+            } else {
+                cell_update(sc, q, ix, iy, iz, dt, out);
+            }
+            g.gv[c] = out;
+            if (keep) keep[c] = q;  // the P2G total, for the reverse pass
+            if (zero_after) {
+                if (flag & FLAG_GMP) g.gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (lane == 0) g.tile_flag[t] = flag & FLAG_STATIC;
+            } else {
+                g.gmp[c] = q;  // the last P2G stays readable (read_grid_mass_momentum)
+                if (lane == 0) g.tile_flag[t] = flag | FLAG_GMP;
+            }
         }
-        g.gv[c] = out;
-        if (keep) keep[c] = q;  // the P2G total, for the reverse pass
-        if (zero_after) {
-            if (flag & FLAG_GMP) g.gmp[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (lane == 0) g.tile_flag[t] = flag & FLAG_STATIC;
-        } else {
-            g.gmp[c] = q;  // the last P2G stays readable (read_grid_mass_momentum)
-            if (lane == 0) g.tile_flag[t] = flag | FLAG_GMP;
-        }
+        if (SPLIT && use_partials) __syncthreads();  // wsum is reused by the next tile
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_count) *next_count = 0u;
 }
@@ -586,14 +618,19 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
 hipError_t launch_grid_update(const SceneDev* scenes, const GridDev& g, const uint32_t* dlist,
                               const uint32_t* dcount, uint32_t* next_count, float dt,
                               int zero_after, int use_partials, int blocks, hipStream_t s, float4* keep,
-                              const RigidDev* rigid) {
+                              const RigidDev* rigid, int split) {
     const RigidDev rg = rigid ? *rigid : RigidDev{nullptr, nullptr, 0u, 0u};
-    if (rg.count)
-        hipLaunchKernelGGL(k_grid_update<true>, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count,
-                           dt, zero_after, use_partials, keep, rg);
-    else
-        hipLaunchKernelGGL(k_grid_update<false>, dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count,
-                           dt, zero_after, use_partials, keep, rg);
+#define MPM_GRID(R, S)                                                                                          \
+    hipLaunchKernelGGL((k_grid_update<R, S>), dim3(blocks), dim3(256), 0, s, scenes, g, dlist, dcount, next_count, \
+                       dt, zero_after, use_partials, keep, rg)
+    if (rg.count) {
+        if (split) MPM_GRID(true, true);
+        else MPM_GRID(true, false);
+    } else {
+        if (split) MPM_GRID(false, true);
+        else MPM_GRID(false, false);
+    }
+#undef MPM_GRID
     return hipGetLastError();
 }
 

@@ -778,8 +778,12 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
 //      candidate per lane, whose boxes load in parallel,
 //   3. a ballot keeps the boxes overlapping the tile, and the wave adds
 //      their partials in ascending order, eight loads in flight at a time.
+// part / nparts: this wave adds only the part-th of nparts contiguous ranges
+// of the candidate list, in ascending order, starting from q (the waves of a
+// block can share a tile's candidates; the caller adds their sums in part
+// order).  nparts = 1: the whole list.
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
-                                             int ix, int iy, int iz, float4& q) {
+                                             int ix, int iy, int iz, float4& q, int part = 0, int nparts = 1) {
     const int lane = threadIdx.x & 63;
     const int W = int(sc.W);
     const int X0 = tx * 4, Y0 = ty * 4, Z0 = tz * 4;
@@ -808,10 +812,11 @@ __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& s
         last = max(last, a1);
     }
     off[9] = total;
-    for (int base = 0; base < total; base += 64) {
+    const int beg = total * part / nparts, end = total * (part + 1) / nparts;
+    for (int base = beg; base < end; base += 64) {
         const int j = base + lane;
         int mine = -1;
-        if (j < total) {
+        if (j < end) {
 #pragma unroll
             for (int r = 0; r < 9; ++r)
                 if (j >= off[r] && j < off[r + 1]) mine = c0[r] + (j - off[r]);

@@ -433,6 +433,28 @@ def test_launch_configurations_match_fp64_oracle_and_repeat_bitwise(ppt, seg):
         assert np.array_equal(a[f], b[f]), f
 
 
+def test_grid_update_launch_shapes_agree_and_repeat():
+    # The grid update runs one wave per tile, or for small scenes (at most 64
+    # particles per grid-update block) the block's four waves over four
+    # ranges of each tile's candidate chunks.  Both shapes match the oracle
+    # and each other to f32 rounding, and each repeats bitwise.  512 blocks
+    # put the 8,192-particle cube in the split shape, 64 in the other.
+    cfg, x, meta = workloads.elastic_cube(8192, seed=8)
+    res = {}
+    for gb in (64, 512):
+        launch = {"grid_blocks": gb, "sort_every": 8}
+        _, res[gb] = run_gpu(cfg, x, None, meta["dt"], 40, launch=launch)
+        _, again = run_gpu(cfg, x, None, meta["dt"], 40, launch=launch)
+        for f in ("x", "v", "C", "F"):
+            assert np.array_equal(res[gb][f], again[f]), (gb, f)
+    for f in ("x", "v", "F"):
+        e = rel_err(res[64][f], res[512][f])
+        log(f"  split vs one wave per tile {f}: {e:.2e}")
+        assert e < {"x": 1e-6, "v": 1e-3, "F": 1e-5}[f]
+    errs = _compare_with_oracle(cfg, x, meta["dt"], 40, tag="split grid update", launch={"grid_blocks": 512})
+    _assert_errs(errs, {"x": 1e-6, "v": 1e-3, "F": 1e-5, "C": 1e-3})
+
+
 def test_launch_configuration_round_trips_and_rejects_bad_values():
     cfg, x, meta = workloads.elastic_cube(1024, seed=7)
     sim = MpmSimulation(cfg, max_particles=len(x))
