@@ -70,6 +70,8 @@ struct mpm_sim {
     uint32_t* tile_scene = nullptr;
     uint32_t* tile_lb = nullptr;
     uint32_t* slist = nullptr;
+    int* srec = nullptr;  // per static tile: id, scene, candidate chunk ranges (dense grids)
+    bool srec_valid = false;  // srec describes the current static list and chunk list
     uint32_t* src_slot = nullptr;
     int4* chunk_box = nullptr;
     float4* part = nullptr;
@@ -260,7 +262,7 @@ void release(mpm_sim* s) {
                     s->d_chunks, s->affine, s->staging, s->tile_lb, s->slist,
                     s->src_slot, s->chunk_box, s->part, s->schunks, s->chunk_of_slot, s->cflags,
                     s->cexcl, s->scan_scratch, s->smark, s->spos, s->sscan, s->tile_slot, s->tile_mark,
-                    s->slot_tile, s->defer_list};
+                    s->slot_tile, s->defer_list, s->srec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (float* c : s->ckpt) (void)hipFree(c);
@@ -317,6 +319,7 @@ mpm::GridDev griddev(mpm_sim* s) {
     g.part = s->part;
     g.slist = s->slist;
     g.scount = s->status + mpm::ST_SCOUNT;
+    g.srec = s->sorted && s->srec_valid ? s->srec : nullptr;
     g.chunk_of_slot = s->chunk_of_slot;
     g.part_capacity = s->part_cap;
     if (s->pool_cap) {
@@ -429,6 +432,7 @@ int ensure_grid(mpm_sim* s) {
     HIP_TRY(s, dalloc(&s->list[1], s->total_tiles));
     HIP_TRY(s, dalloc(&s->tile_lb, (size_t)s->total_tiles + 1));
     HIP_TRY(s, dalloc(&s->slist, s->total_tiles));
+    if (!s->pool_cap) HIP_TRY(s, dalloc(&s->srec, (size_t)s->total_tiles * mpm::SREC_WORDS));
     s->part_cap = s->cap / 256 + s->cap / 64 + s->n_scenes + 64;
     s->max_schunks = s->cap + s->cap / 256 + s->n_scenes + 64;
     HIP_TRY(s, dalloc(&s->chunk_box, 2 * (size_t)s->part_cap));
@@ -512,6 +516,11 @@ int sort_particles(mpm_sim* s, bool reorder) {
         HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
                                             s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
                                             s->chunk_of_slot, CHUNK * s->chunk_ppt, s->seg_tiles, s->stream));
+        s->srec_valid = s->srec != nullptr;
+        if (s->srec)
+            HIP_TRY(s, mpm::launch_static_records(s->d_scenes, s->tile_scene, s->slist, s->status + mpm::ST_SCOUNT,
+                                                  s->total_tiles, s->tile_lb, s->chunk_of_slot, s->part_cap, s->srec,
+                                                  s->stream));
         // previous slot of every new slot: the sorted slots spread over cells
         // inside each chunk, or the reference order's ids mapped to slots
         if (by_slot)

@@ -114,6 +114,7 @@ constexpr uint32_t FLAG_STATIC = 4u;  // on the static active list of the last s
 // last sort) stays well inside it, and a larger box falls back to global
 // atomics (counted in ST_FALLBACK).
 constexpr int LDS_CELLS = 1600;
+constexpr int SREC_WORDS = 20;
 
 struct GridDev {
     // sparse pool (null for a dense grid): 1 + slot per tile (0: none), the
@@ -132,6 +133,10 @@ struct GridDev {
     float4* part;         // per chunk: LDS_CELLS cells (m, px, py, pz)
     const uint32_t* slist;  // static active list
     const uint32_t* scount;
+    // per static-list entry (SREC_WORDS ints, built at the sort; null: none):
+    // tile, scene, then (first, last) candidate chunk of the nine tile rows
+    // around the tile (last < first: none)
+    const int* srec;
     const uint32_t* chunk_of_slot;  // sorted slot -> chunk of the fused pass
     uint32_t part_capacity;         // chunks with a partial slot
 };
@@ -220,6 +225,11 @@ hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_sc
                                  uint32_t total_tiles, uint32_t* mark, uint32_t* pos, uint32_t* scan_scratch,
                                  hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// The static tiles' records (GridDev::srec) for the sorted chunk list:
+// max_static bounds the static count (read on the device).
+hipError_t launch_static_records(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* slist,
+                                 const uint32_t* scount, uint32_t max_static, const uint32_t* tile_lb,
+                                 const uint32_t* chunk_of_slot, uint32_t part_capacity, int* srec, hipStream_t s);
 hipError_t launch_tile_keys(const StepArgs& a, uint32_t* keys_out, int by_slot, hipStream_t s);
 hipError_t launch_chunk_spread(const SceneDev* scenes, const Chunk* chunks, const uint32_t* n_chunks,
                                uint32_t max_chunks, const float* ps, uint32_t cap, const uint32_t* src_in,

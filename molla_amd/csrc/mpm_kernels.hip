@@ -559,9 +559,18 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
     const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
     const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
     for (uint32_t i = first; i < walk.n; i += stride) {
-        uint32_t t, st;
-        walk.at(g, dlist, i, t, st);
-        const SceneDev sc = scenes[g.tile_scene[t]];
+        uint32_t t, st, scene;
+        // a static tile's record holds its id, scene and candidate ranges
+        const int* rec = g.srec && i < walk.ns ? g.srec + (size_t)i * SREC_WORDS : nullptr;
+        if (rec) {
+            t = uint32_t(rec[0]);
+            st = t;
+            scene = uint32_t(rec[1]);
+        } else {
+            walk.at(g, dlist, i, t, st);
+            scene = g.tile_scene[t];
+        }
+        const SceneDev sc = scenes[scene];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (st << 6) | uint32_t(lane);
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
         // range 0 starts from the atomic channel
         float4 q = (flag & FLAG_GMP) && (!SPLIT || wave == 0) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (use_partials) {
-            add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? GRID_RANGES : 1);
+            add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? GRID_RANGES : 1, rec);
             if (SPLIT) {
                 if (wave > 0) wsum[wave - 1][lane] = q;
                 __syncthreads();
@@ -936,6 +945,52 @@ hipError_t launch_static_rebuild(const SceneDev* scenes, const uint32_t* tile_sc
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// One thread per (static tile, tile row): the row's candidate chunk range, as
+// add_partials would find it from the tile bounds and the chunk of each slot.
+__global__ void k_static_records(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                                 const uint32_t* __restrict__ slist, const uint32_t* __restrict__ scount,
+                                 uint32_t max_static, const uint32_t* __restrict__ lb,
+                                 const uint32_t* __restrict__ chunk_of_slot, uint32_t part_capacity,
+                                 int* __restrict__ srec) {
+    const uint32_t ns = min(*scount, max_static);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < ns * 9u; j += gridDim.x * blockDim.x) {
+        const uint32_t i = j / 9u, r = j % 9u;
+        const uint32_t t = slist[i], scene = tile_scene[t];
+        const SceneDev& sc = scenes[scene];
+        const uint32_t tl = t - sc.tile_base;
+        const int W = int(sc.W);
+        const int tx = int(tl % sc.W), ty = int((tl / sc.W) % sc.W), tz = int(tl / (sc.W * sc.W));
+        const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
+        const int nz = tz + int(r) / 3 - 1, ny = ty + int(r) % 3 - 1;
+        int c0 = 0, c1 = -1;
+        if (nz >= 0 && nz < W && ny >= 0 && ny < W) {
+            const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));
+            const uint32_t s0 = lb[row + xa], s1 = lb[row + xb + 1];
+            if (s0 < s1) {
+                c0 = int(chunk_of_slot[s0]);
+                c1 = min(int(chunk_of_slot[s1 - 1]), int(part_capacity) - 1);
+            }
+        }
+        int* rec = srec + (size_t)i * SREC_WORDS;
+        if (r == 0) {
+            rec[0] = int(t);
+            rec[1] = int(scene);
+        }
+        rec[2 + 2 * r] = c0;
+        rec[3 + 2 * r] = c1;
+    }
+}
+
+hipError_t launch_static_records(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* slist,
+                                 const uint32_t* scount, uint32_t max_static, const uint32_t* tile_lb,
+                                 const uint32_t* chunk_of_slot, uint32_t part_capacity, int* srec, hipStream_t s) {
+    if (max_static == 0) return hipSuccess;
+    const unsigned blocks = std::max(1u, std::min<unsigned>((max_static * 9u + 255u) / 256u, 8192u));
+    hipLaunchKernelGGL(k_static_records, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, slist, scount,
+                       max_static, tile_lb, chunk_of_slot, part_capacity, srec);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {

@@ -781,15 +781,22 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
 // part / nparts: this wave adds only the part-th of nparts contiguous ranges
 // of the candidate list, in ascending order, starting from q (the waves of a
 // block can share a tile's candidates; the caller adds their sums in part
-// order).  nparts = 1: the whole list.
+// order).  nparts = 1: the whole list.  rec (may be null): the tile's static
+// record (GridDev::srec), which holds the nine rows' candidate ranges.
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
-                                             int ix, int iy, int iz, float4& q, int part = 0, int nparts = 1) {
+                                             int ix, int iy, int iz, float4& q, int part = 0, int nparts = 1,
+                                             const int* rec = nullptr) {
     const int lane = threadIdx.x & 63;
     const int W = int(sc.W);
     const int X0 = tx * 4, Y0 = ty * 4, Z0 = tz * 4;
     const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
     int rc0 = 0, rc1 = -1;  // empty row
-    if (lane < 9) {
+    if (rec) {
+        if (lane < 9) {
+            rc0 = rec[2 + 2 * lane];
+            rc1 = rec[3 + 2 * lane];
+        }
+    } else if (lane < 9) {
         const int nz = tz + lane / 3 - 1, ny = ty + lane % 3 - 1;
         if (nz >= 0 && nz < W && ny >= 0 && ny < W) {
             const uint32_t row = sc.tile_base + uint32_t(W * (ny + W * nz));

@@ -103,6 +103,49 @@ def test_p2g_with_stress_matches_oracle():
         assert em < max(4 * em32, 1e-6) and ep < max(4 * ep32, 1e-6)
 
 
+def test_p2g_mixes_fast_and_resting_particles_within_one_chunk():
+    # The fused scatter's momentum words are fixed point at 2^-23 of the
+    # chunk's largest single contribution bound, so a cell fed only by slow
+    # particles next to a fast one (in the same chunk) is rounded to that
+    # quantum rather than to f32 of its own value.  The bound per cell is
+    # N_contributions x 2^-23 x (the chunk's bound): here one particle at
+    # 40 m/s among 1,000 at ~1 mm/s, all in one tile, compared with the FP64
+    # P2G cell by cell.  The mass words (2^-40 of the largest particle mass)
+    # stay at f32-scatter accuracy.
+    cfg = MpmConfig.sand().copy(youngs=2.0e4, rheology=Viscous(0, 0), gravity_y=0.0, grid_origin=(0.0, 0.0, 0.0),
+                                grid_size=32, cell_size=1 / 32)
+    rng = np.random.default_rng(17)
+    n = 1000
+    x = (np.array([0.5, 0.5, 0.5]) + (rng.random((n, 3)) - 0.5) * (4 / 32)).astype(np.float32)  # one 4^3 tile
+    v = (1e-3 * rng.standard_normal((n, 3))).astype(np.float32)
+    v[0] = (40.0, 0.0, 0.0)
+    sim = MpmSimulation(cfg, max_particles=n)
+    sim.set_particles(x, v)
+    sim.step_n(1e-5, 1)
+    m, p = sim.read_grid_mass_momentum()
+    st = sim.read_status()
+    sim.close()
+    assert st["escapes"] == 0 and st["fallback_chunks"] == 0
+    gm, gp = orc.p2g(cfg, oracle_state(cfg, x, v), 1e-5)
+    m32, p32 = orc.step(cfg, oracle_state(cfg, x, v, dtype=np.float32), 1e-5, 1, want_grid=True)
+    em, em32 = rel_err(m, gm), rel_err(m32, gm)
+    log(f"  cell mass err {em:.2e} (f32 scatter {em32:.2e})")
+    assert em < max(4 * em32, 1e-6)
+    # momentum: per cell within the fixed-point bound; the bound on any one
+    # contribution is |m v| + 1.5 dx |A| <= 40 m (no stress, C = 0)
+    bound = 40.0 * cfg.particle_mass
+    quantum = bound * 2.0 ** -23
+    err = np.abs(p.astype(np.float64) - gp).max()
+    log(f"  momentum max abs err {err:.3e} = {err / quantum:.1f} quanta of 2^-23 x the chunk bound "
+        f"(f32 scatter {np.abs(p32.astype(np.float64) - gp).max():.3e})")
+    assert err <= 64 * quantum  # a cell takes at most a few hundred contributions, each rounded by <= 1/2 quantum
+    # the slow cells' velocities (p / m) within that bound over their own mass
+    slow = (gm > 1e-3 * gm.max()) & (np.abs(gp[:, 0]) < 1e-2 * bound)
+    dv = np.abs(p[slow].astype(np.float64) / m[slow, None] - gp[slow] / gm[slow, None]).max(axis=1)
+    log(f"  slow cells: {int(slow.sum())}, max velocity error {dv.max():.2e} m/s (fast particle 40 m/s)")
+    assert slow.sum() > 50 and np.all(dv <= 64 * quantum / gm[slow])
+
+
 # ---------------------------------------------------------------- whole steps
 def test_single_particle_free_fall_is_exact():
     # tests/mpm_freefall_regression_gpu.rs:76-111
