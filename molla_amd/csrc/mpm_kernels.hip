@@ -547,8 +547,13 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
 // count and the launch size only, so a run repeats bitwise; the two shapes
 // agree to f32 rounding.
 constexpr int GRID_RANGES = 4;
+// One wave per tile runs at seven waves per SIMD with four partial loads in
+// flight (68 VGPRs): with ten thousand tiles or more, more tiles in flight
+// beat deeper load batches.  The split shape keeps eight loads in flight
+// (dense small scenes have ~100 overlapping chunks per tile).
 template <bool RIGID, bool SPLIT>
-__global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict__ scenes, GridDev g,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 : (SPLIT ? 5 : 7)))) void k_grid_update(
+    const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
                                                      uint32_t* __restrict__ next_count, float dt,
@@ -578,7 +583,8 @@ __global__ __launch_bounds__(256) void k_grid_update(const SceneDev* __restrict_
         // range 0 starts from the atomic channel
         float4 q = (flag & FLAG_GMP) && (!SPLIT || wave == 0) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (use_partials) {
-            add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? GRID_RANGES : 1, rec);
+            add_partials<(SPLIT || RIGID) ? 8 : 4>(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0,
+                                                   SPLIT ? GRID_RANGES : 1, rec);
             if (SPLIT) {
                 if (wave > 0) wsum[wave - 1][lane] = q;
                 __syncthreads();

@@ -777,12 +777,14 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
 //   2. the deduplicated ranges are laid out as one candidate list, one
 //      candidate per lane, whose boxes load in parallel,
 //   3. a ballot keeps the boxes overlapping the tile, and the wave adds
-//      their partials in ascending order, eight loads in flight at a time.
+//      their partials in ascending order, U loads in flight at a time.
 // part / nparts: this wave adds only the part-th of nparts contiguous ranges
 // of the candidate list, in ascending order, starting from q (the waves of a
 // block can share a tile's candidates; the caller adds their sums in part
 // order).  nparts = 1: the whole list.  rec (may be null): the tile's static
 // record (GridDev::srec), which holds the nine rows' candidate ranges.
+// U: overlapping chunks whose partial loads are in flight together.
+template <int U = 8>
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
                                              int ix, int iy, int iz, float4& q, int part = 0, int nparts = 1,
                                              const int* rec = nullptr) {
@@ -838,12 +840,8 @@ __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& s
         }
         uint64_t m = __ballot(hit);
         while (m) {
-            // up to eight overlapping chunks per round: their loads are all
-            // in flight before the (ascending-order) adds (dense scenes have
-            // tens of overlapping chunks per tile; four per round at eight
-            // waves per SIMD was 1 us faster on the 2 M scene and 3 us slower
-            // on the 65,536-particle diff scene)
-            constexpr int U = 8;
+            // up to U overlapping chunks per round: their loads are all in
+            // flight before the (ascending-order) adds
             int ls[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
