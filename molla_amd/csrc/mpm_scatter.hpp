@@ -751,16 +751,33 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
     d = a.chunk_box[2 * (size_t)ci + 1];
     if (d.x <= 0 || d.y <= 0 || d.z <= 0 || d.x * d.y * d.z > LDS_CELLS) return false;
     float4* stage = reinterpret_cast<float4*>(&L.acc[0][0]);
-    const int rows = d.y * d.z;
-    const float inv_by = 1.0f / float(d.y);
-    // 16 lanes along x, 16 rows (y, z) per pass
-    for (int r = int(threadIdx.x >> 4); r < rows; r += BLOCK / 16) {
-        const int lz = int((float(r) + 0.5f) * inv_by), ly = r - lz * d.y;
-        for (int lx = int(threadIdx.x & 15); lx < d.x; lx += 16) {
-            uint32_t c;
-            stage[lx + d.x * r] = cell_storage(sc, SPARSE ? a.tile_slot : nullptr, o.x + lx, o.y + ly, o.z + lz, c)
-                                      ? a.gv[c]
-                                      : make_float4(0.f, 0.f, 0.f, 0.f);
+    // Box cell i = lx + bx (ly + by lz) of thread t: i = t + u BLOCK.  All of
+    // a thread's loads are issued before its LDS stores, so the staging costs
+    // one load latency rather than one per cell (nothing of the particle
+    // pass is live yet, so the registers are free).
+    constexpr int PER = 4;  // loads in flight per thread
+    const int vol = d.x * d.y * d.z;
+    const float inv_bx = 1.0f / float(d.x), inv_by = 1.0f / float(d.y);
+#pragma unroll 1
+    for (int i0 = 0; i0 < vol; i0 += PER * BLOCK) {
+        float4 v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = i0 + int(threadIdx.x) + u * BLOCK;
+            v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < vol) {
+                // exact for the box's small integers: (n + 0.5) / b is never
+                // within f32 rounding of an integer
+                const int r = int((float(i) + 0.5f) * inv_bx), lx = i - r * d.x;
+                const int lz = int((float(r) + 0.5f) * inv_by), ly = r - lz * d.y;
+                uint32_t c;
+                if (cell_storage(sc, SPARSE ? a.tile_slot : nullptr, o.x + lx, o.y + ly, o.z + lz, c)) v[u] = a.gv[c];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int i = i0 + int(threadIdx.x) + u * BLOCK;
+            if (i < vol) stage[i] = v[u];
         }
     }
     __syncthreads();
