@@ -204,10 +204,15 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks)
  *                                 pass takes per chunk of the sorted layout
  *                                 (a chunk holds 256 x this many; shader
  *                                 model only, MPM_MODEL_CPUDIFF handles run 1);
- *   segment_tiles                 4 or 8: the longest run of tiles along x
- *                                 one chunk may span, which bounds the LDS
- *                                 box of its scatter (8 only where the group
- *                                 of eight tiles holds two chunks' worth).
+ *   segment_tiles                 the chunk rule of the sorted layout; both
+ *                                 keep a chunk inside one tile row, which
+ *                                 bounds the LDS box of its scatter.  4
+ *                                 (default): greedy rows, a chunk runs for
+ *                                 256 x particles_per_thread particles or
+ *                                 until it would span more than four tiles;
+ *                                 8: aligned groups of eight tiles where
+ *                                 they hold two chunks' worth, else of four,
+ *                                 cut every 256 x particles_per_thread.
  * The block is 256 threads (4 waves): with the 168-VGPR cap of three waves
  * per SIMD and three ~49 KB LDS boxes per CU, it is the size that fills a
  * CU.  Changing particles_per_thread or segment_tiles re-sorts at the next

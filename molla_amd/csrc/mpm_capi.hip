@@ -123,7 +123,7 @@ struct mpm_sim {
     uint32_t pblocks = 0;
     uint32_t gblocks = 0;
     uint32_t ppt = 2;         // particles per thread asked for (shader model)
-    uint32_t seg_tiles = 8;   // longest chunk segment, tiles along x
+    uint32_t seg_tiles = 4;   // chunk rule: 4 greedy rows of at most four tiles, 8 aligned segments
     uint32_t chunk_ppt = 1;   // particles per thread of the current sorted chunk list
     // profiling
     int profiling = 0;         // MPM_PROFILE_* flags
@@ -515,7 +515,8 @@ int sort_particles(mpm_sim* s, bool reorder) {
         s->chunk_ppt = (by_slot && s->sh[0].model == MPM_MODEL_SHADER) ? s->ppt : 1u;
         HIP_TRY(s, mpm::build_sorted_chunks(s->d_scenes, s->tile_scene, s->keys, s->tile_lb, s->n, s->cflags,
                                             s->cexcl, s->scan_scratch, s->schunks, s->status + mpm::ST_NCHUNKS,
-                                            s->chunk_of_slot, CHUNK * s->chunk_ppt, s->seg_tiles, s->stream));
+                                            s->chunk_of_slot, CHUNK * s->chunk_ppt, s->seg_tiles, s->total_tiles,
+                                            s->stream));
         s->srec_valid = s->srec != nullptr;
         if (s->srec)
             HIP_TRY(s, mpm::launch_static_records(s->d_scenes, s->tile_scene, s->slist, s->status + mpm::ST_SCOUNT,

@@ -297,7 +297,8 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
                                const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
-                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, hipStream_t s);
+                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, uint32_t total_tiles,
+                               hipStream_t s);
 
 // ---- radix sort (mpm_sort.hip) ----
 size_t radix_scratch_words(uint32_t n);

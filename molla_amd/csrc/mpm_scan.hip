@@ -122,9 +122,22 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 }
 
 // ---------------------------------------------------------------------------
-// Chunk list of the sorted particle array.
-// The segment of tile `key`: its first tile and its length along x.  lb is
-// the tile lower-bound table of the sorted keys.
+// Chunk list of the sorted particle array.  Two rules, both keeping a chunk
+// inside one tile row (so its scatter box is at most the row's 4 x 4 cross
+// section plus the stencil reach in y and z):
+//   greedy rows (span tiles, the default with span 4): a chunk starts at
+//     each row's first particle and runs for chunk_cap particles or until
+//     the next particle would make it span more than `span` tiles, so chunks
+//     start anywhere inside a tile and only a row's last chunk (or one cut by
+//     the span) is partly filled;
+//   aligned segments: chunks of chunk_cap from the start of each aligned
+//     group of four tiles, or of eight where those eight hold at least two
+//     chunks' worth.
+// The greedy rule makes ~10 % fewer chunks on the 2 M scene and ~15 % fewer
+// on the 512-scene batch, with fewer partly filled blocks.
+//
+// The aligned segment of tile `key`: its first tile and its length along x.
+// lb is the tile lower-bound table of the sorted keys.
 // dense_group: particles that make an eight-tile group one segment (~0u: never).
 __device__ __forceinline__ void segment_of(const SceneDev& sc, const uint32_t* lb, uint32_t key, uint32_t dense_group,
                                            uint32_t& first, uint32_t& ntiles) {
@@ -155,9 +168,37 @@ __global__ void k_chunk_flags(const SceneDev* __restrict__ scenes, const uint32_
     }
 }
 
+// Greedy rows: one thread per tile row marks its chunks' first slots in
+// `flags` (zero elsewhere, cleared before).  Each step is one table lookup:
+// the chunk ends at chunk_cap particles or at the first slot of the tile
+// `span` tiles past the tile holding its first particle, whichever is first.
+__global__ void k_row_chunks(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
+                             const uint32_t* __restrict__ lb, uint32_t total_tiles, uint32_t chunk_cap, uint32_t span,
+                             uint32_t* __restrict__ flags) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total_tiles; t += gridDim.x * blockDim.x) {
+        const SceneDev& sc = scenes[tile_scene[t]];
+        if ((t - sc.tile_base) % sc.W) continue;  // not a row's first tile
+        const uint32_t row_end = t + sc.W;        // one past the row's last tile
+        const uint32_t e_row = lb[row_end];
+        uint32_t s = lb[t], tt = t;
+        while (s < e_row) {
+            flags[s] = 1u;
+            while (lb[tt + 1] <= s) ++tt;  // the tile holding slot s
+            s = min(s + chunk_cap, lb[min(tt + span, row_end)]);
+        }
+    }
+}
+
+// Greedy rows: a chunk's count is the distance to the next chunk's start.
+__global__ void k_chunk_counts(Chunk* __restrict__ chunks, const uint32_t* __restrict__ nch, uint32_t n) {
+    const uint32_t m = *nch;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < m; c += gridDim.x * blockDim.x)
+        chunks[c].count = (c + 1 < m ? chunks[c + 1].start : n) - chunks[c].start;
+}
+
 __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t* __restrict__ tile_scene,
                              const uint32_t* __restrict__ keys, const uint32_t* __restrict__ lb, uint32_t n,
-                             uint32_t chunk_cap, uint32_t dense_group, const uint32_t* __restrict__ flags,
+                             uint32_t chunk_cap, uint32_t dense_group, bool greedy, const uint32_t* __restrict__ flags,
                              const uint32_t* __restrict__ excl,
                              Chunk* __restrict__ chunks, uint32_t* __restrict__ chunk_of_slot) {
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -167,13 +208,15 @@ __global__ void k_chunk_emit(const SceneDev* __restrict__ scenes, const uint32_t
         const uint32_t key = keys[j];
         const uint32_t scn = tile_scene[key];
         const SceneDev& sc = scenes[scn];
-        uint32_t first, nt;
-        segment_of(sc, lb, key, dense_group, first, nt);
-        const uint32_t seg_end = lb[first + nt];
         Chunk ch;
         ch.scene = scn;
         ch.start = j;
-        ch.count = min(chunk_cap, seg_end - j);
+        ch.count = 0u;  // greedy rows: k_chunk_counts
+        if (!greedy) {
+            uint32_t first, nt;
+            segment_of(sc, lb, key, dense_group, first, nt);
+            ch.count = min(chunk_cap, lb[first + nt] - j);
+        }
         // the tile row of the chunk's segment (read by stencil_escapes)
         const uint32_t r = (key - sc.tile_base) / sc.W;
         ch.pad = chunk_row_tag(r % sc.W, r / sc.W);
@@ -192,18 +235,29 @@ hipError_t launch_copy_u32(const uint32_t* src, uint32_t* dst, hipStream_t s) {
 hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scene, const uint32_t* keys,
                                const uint32_t* lb, uint32_t n, uint32_t* flags, uint32_t* excl,
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
-                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, hipStream_t s) {
+                               uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, uint32_t total_tiles,
+                               hipStream_t s) {
     if (chunk_cap == 0 || chunk_cap > CHUNK_PARTICLES * MAX_PPT || (seg_tiles != 4 && seg_tiles != 8))
         return hipErrorInvalidValue;
-    const uint32_t dense_group = seg_tiles == 8 ? 2 * chunk_cap : ~0u;
+    const bool greedy = seg_tiles == 4;
+    const uint32_t dense_group = 2 * chunk_cap;
     const unsigned blocks = std::max(1u, std::min<unsigned>((n + 255) / 256, 8192u));
-    hipLaunchKernelGGL(k_chunk_flags, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, chunk_cap,
-                       dense_group, flags);
+    if (greedy) {
+        hipError_t e = launch_fill_u32(flags, 0u, n, s);
+        if (e != hipSuccess) return e;
+        const unsigned tb = std::max(1u, std::min<unsigned>((total_tiles + 255) / 256, 8192u));
+        hipLaunchKernelGGL(k_row_chunks, dim3(tb), dim3(256), 0, s, scenes, tile_scene, lb, total_tiles, chunk_cap,
+                           seg_tiles, flags);
+    } else {
+        hipLaunchKernelGGL(k_chunk_flags, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, chunk_cap,
+                           dense_group, flags);
+    }
     uint32_t* total = nullptr;
     hipError_t e = exclusive_scan_u32(flags, excl, n, scan_scratch, &total, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_chunk_emit, dim3(blocks), dim3(256), 0, s, scenes, tile_scene, keys, lb, n, chunk_cap,
-                       dense_group, flags, excl, chunks, chunk_of_slot);
+                       dense_group, greedy, flags, excl, chunks, chunk_of_slot);
+    if (greedy) hipLaunchKernelGGL(k_chunk_counts, dim3(blocks), dim3(256), 0, s, chunks, total, n);
     hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, s, total, n_chunks);
     return hipGetLastError();
 }

@@ -531,6 +531,65 @@ def test_launch_configuration_round_trips_and_rejects_bad_values():
     ref.close()
 
 
+def _model_chunk_count(tile_counts: np.ndarray, W: int, cap: int, segment_tiles: int) -> int:
+    """Chunks the sorted layout's chunk rule makes for one scene whose tiles
+    (x fastest) hold tile_counts particles.  segment_tiles 4: greedy rows (a
+    chunk runs for cap particles or until it would span more than four
+    tiles); 8: aligned segments of eight tiles holding at least 2 cap
+    particles, else of four, cut every cap particles."""
+    n = 0
+    for row in range(W * W):
+        c = tile_counts[row * W:(row + 1) * W]
+        if not c.any():
+            continue
+        lb = np.concatenate([[0], np.cumsum(c)])
+        if segment_tiles == 4:
+            s, tt = 0, 0
+            while s < lb[W]:
+                n += 1
+                while lb[tt + 1] <= s:
+                    tt += 1
+                s = min(s + cap, lb[min(tt + 4, W)])
+        else:
+            for g8 in range(0, W, 8):
+                n8 = min(8, W - g8)
+                tot8 = lb[g8 + n8] - lb[g8]
+                if tot8 >= 2 * cap:
+                    n += -(-tot8 // cap)
+                else:
+                    for g4 in range(g8, g8 + n8, 4):
+                        n += -(-(lb[min(g4 + 4, W)] - lb[g4]) // cap)
+    return n
+
+
+@pytest.mark.parametrize("ppt", [1, 2])
+def test_chunk_rules_make_the_modelled_chunk_count(ppt):
+    # The sort's chunk list, counted after the first (sorting) substep, equals
+    # a row-by-row model of each rule on the oracle's tile keys: one scene and
+    # a ten-scene batch of mixed sizes.
+    cfgs, xs, meta = workloads.batched_scenes(10, seed=11)
+    cube_cfg, cube_x, cube_meta = workloads.elastic_cube(8192, seed=3)
+    for case, (cs, ps, dt) in {"cube": ([cube_cfg], [cube_x], cube_meta["dt"]),
+                               "batch": (cfgs, xs, meta["dt"])}.items():
+        for seg in (4, 8):
+            want = 0
+            for c, x in zip(cs, ps):
+                W = (c.grid_size + 3) // 4
+                keys, _ = orc.tile_keys(c, x)
+                want += _model_chunk_count(np.bincount(keys.astype(np.int64), minlength=W ** 3), W, 256 * ppt, seg)
+            sim = MpmSimulation(cs) if len(cs) > 1 else MpmSimulation(cs[0], max_particles=len(ps[0]))
+            if len(cs) > 1:
+                sim.set_particles_batch(ps)
+            else:
+                sim.set_particles(ps[0], None)
+            sim.set_launch_config(particles_per_thread=ppt, segment_tiles=seg)
+            sim.step_n(dt, 1)
+            st = sim.read_status()
+            sim.close()
+            log(f"  {case} ppt {ppt} segment_tiles {seg}: {st['chunks']} chunks (model {want})")
+            assert st["sorts"] == 1 and st["chunks"] == want
+
+
 def test_autotune_restores_state_and_keeps_results():
     cfg, x, meta = workloads.elastic_cube(8192, seed=5)
     dt = meta["dt"]
