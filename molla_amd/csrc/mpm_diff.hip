@@ -404,7 +404,7 @@ __device__ __forceinline__ void adj_p2g_particle(const StepArgs& a, const float4
 // of the block calls it.
 __device__ __forceinline__ void add_chunk_mu_lambda(double* gml, uint32_t ci, double amu, double alam,
                                                     double (*red)[WAVES]) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     amu = warp_sum(amu);
     alam = warp_sum(alam);
     if (lane == 0) {
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ s
     constexpr int R = 4;
     __shared__ float4 wsum[SPLIT ? R - 1 : 1][64];
     const uint32_t ns = *g.scount, nd = *dcount;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
     const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
     for (uint32_t i = first; i < ns + nd; i += stride) {

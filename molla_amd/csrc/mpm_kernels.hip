@@ -547,12 +547,13 @@ __device__ __forceinline__ void cell_update(const SceneDev& sc, float4 q, int ix
 // count and the launch size only, so a run repeats bitwise; the two shapes
 // agree to f32 rounding.
 constexpr int GRID_RANGES = 4;
-// One wave per tile runs at seven waves per SIMD with four partial loads in
-// flight (68 VGPRs): with ten thousand tiles or more, more tiles in flight
-// beat deeper load batches.  The split shape keeps eight loads in flight
-// (dense small scenes have ~100 overlapping chunks per tile).
+// One wave per tile runs at eight waves per SIMD with four partial loads in
+// flight (51 VGPRs, the tile's record and scene in scalar registers): with
+// ten thousand tiles or more, more tiles in flight beat deeper load batches.
+// The split shape keeps eight loads in flight (dense small scenes have ~100
+// overlapping chunks per tile).
 template <bool RIGID, bool SPLIT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 : (SPLIT ? 5 : 7)))) void k_grid_update(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 : (SPLIT ? 5 : 8)))) void k_grid_update(
     const SceneDev* __restrict__ scenes, GridDev g,
                                                      const uint32_t* __restrict__ dlist,
                                                      const uint32_t* __restrict__ dcount,
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 :
                                                      int zero_after, int use_partials, float4* keep, RigidDev rg) {
     __shared__ float4 wsum[SPLIT ? GRID_RANGES - 1 : 1][64];
     const TileWalk walk(g, dcount);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
     const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
     for (uint32_t i = first; i < walk.n; i += stride) {
@@ -659,7 +660,7 @@ __global__ __launch_bounds__(256) void k_rigid_shell(const SceneDev* __restrict_
                                                      const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
     const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+    for (uint32_t i = blockIdx.x * 4 + uint32_t(wave_id()); i < walk.n; i += gridDim.x * 4) {
         uint32_t t, st;
         walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
@@ -720,7 +721,7 @@ __global__ __launch_bounds__(256) void k_implicit_sweep(const SceneDev* __restri
                                                         const uint32_t* __restrict__ dcount, float dt, RigidDev rg) {
     const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+    for (uint32_t i = blockIdx.x * 4 + uint32_t(wave_id()); i < walk.n; i += gridDim.x * 4) {
         uint32_t t, st;
         walk.at(g, dlist, i, t, st);
         const SceneDev sc = scenes[g.tile_scene[t]];
@@ -781,7 +782,7 @@ __global__ __launch_bounds__(256) void k_copy_tiles(GridDev g, const float4* __r
                                                     const uint32_t* __restrict__ dcount) {
     const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+    for (uint32_t i = blockIdx.x * 4 + uint32_t(wave_id()); i < walk.n; i += gridDim.x * 4) {
         uint32_t t, st;
         walk.at(g, dlist, i, t, st);
         dst[(st << 6) | uint32_t(lane)] = src[(st << 6) | uint32_t(lane)];
@@ -815,7 +816,7 @@ __global__ __launch_bounds__(256) void k_clear_tiles(GridDev g, const uint32_t* 
                                                      const uint32_t* __restrict__ dcount, float4* also) {
     const TileWalk walk(g, dcount);
     const int lane = threadIdx.x & 63;
-    for (uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6); i < walk.n; i += gridDim.x * 4) {
+    for (uint32_t i = blockIdx.x * 4 + uint32_t(wave_id()); i < walk.n; i += gridDim.x * 4) {
         uint32_t t, st;
         walk.at(g, dlist, i, t, st);
         g.gmp[(st << 6) | uint32_t(lane)] = make_float4(0.f, 0.f, 0.f, 0.f);

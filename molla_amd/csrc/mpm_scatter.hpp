@@ -12,6 +12,12 @@ namespace mpm {
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 
+// The wave's index in its block, as a wave-uniform value: the compiler cannot
+// tell that threadIdx.x >> 6 is the same on every lane, and a per-tile index
+// built from it would turn the tile's record and scene loads into vector
+// loads (one dependent VGPR load per field) instead of scalar ones.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6)); }
+
 // XCD-aware walk of a chunk list by a persistent grid.  Workgroups are
 // dispatched round-robin over the 8 XCDs of an MI355X, each with its own L2,
 // so block b runs on XCD b % 8.  Giving every XCD one contiguous range of the
