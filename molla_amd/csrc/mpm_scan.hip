@@ -133,7 +133,7 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 //   aligned segments: chunks of chunk_cap from the start of each aligned
 //     group of four tiles, or of eight where those eight hold at least two
 //     chunks' worth.
-// The greedy rule makes ~10 % fewer chunks on the 2 M scene and ~15 % fewer
+// The greedy rule makes ~10 % fewer chunks on the 2 M scene and ~14 % fewer
 // on the 512-scene batch, with fewer partly filled blocks.
 //
 // The aligned segment of tile `key`: its first tile and its length along x.
