@@ -463,13 +463,14 @@ def test_escaped_particles_fall_back_to_global_atomics():
 
 @pytest.mark.parametrize("ppt,seg", [(1, 4), (1, 8), (2, 4), (2, 8)])
 def test_launch_configurations_match_fp64_oracle_and_repeat_bitwise(ppt, seg):
-    # Particles per thread and segment length change the chunks (256 or 512
-    # particles, runs of 4 or 8 tiles) and with them each chunk's fixed-point
-    # scale, so results differ across configurations at f32 rounding only;
-    # one configuration run twice is bitwise identical.  The cube holds 16
-    # particles per cell, so 8-tile segments and full 512-particle chunks
-    # occur, and with 11 sorts over 60 substeps the sorted chunk lists are
-    # rebuilt under the configuration.
+    # Particles per thread and the chunk rule change the chunks (256 or 512
+    # particles; greedy rows of at most 4 tiles, or aligned 4- and 8-tile
+    # segments) and with them each chunk's fixed-point scale, so results
+    # differ across configurations at f32 rounding only; one configuration
+    # run twice is bitwise identical.  The cube holds 16 particles per cell,
+    # so 8-tile segments and full 512-particle chunks occur, and with 11
+    # sorts over 60 substeps the sorted chunk lists are rebuilt under the
+    # configuration.
     cfg, x, meta = workloads.elastic_cube(8192, seed=6)
     launch = {"particles_per_thread": ppt, "segment_tiles": seg, "sort_every": 6}
     errs = _compare_with_oracle(cfg, x, meta["dt"], 60, tag=f"ppt{ppt} seg{seg}", launch=launch)
