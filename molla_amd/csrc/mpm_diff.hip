@@ -179,7 +179,7 @@ __device__ __forceinline__ void adj_g2p_particle(const StepArgs& a, const SceneD
     for (int k = 0; k < 9; ++k) sp.A[k] = B[k];
     sp.m = 0.f;
     sp.on = true;
-    sp.esc = stencil_escapes(a, sc, ch, slot, sp.b);
+    sp.esc = stencil_escapes(sc, ch, a.sorted_keys[slot], sp.b);
 }
 
 // Reverse of P2G, of the stress and of the SVD for the particle in `slot`

@@ -146,7 +146,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
             }
             bool ok = true;
             float v[3], C[9];
+            // the slot's sorted tile key, for the scatter's escape test; read
+            // before this particle's state stores, which the compiler cannot
+            // move it past
+            uint32_t skey = 0u;
             if (MODE == MODE_P2G) {
+                if (a.use_lds) skey = a.sorted_keys[slot];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) v[k] = plane_ld(ps, cap, CV + k, boff);
 #pragma unroll
@@ -401,6 +406,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                             }
                             if (!plast_ok) latch(a.status, FAULT_NUMERICAL);
                         }
+                        if (MODE != MODE_G2P && a.use_lds) skey = a.sorted_keys[slot];
                         // ---- write the particle state ----
 #pragma unroll
                         for (int k = 0; k < 3; ++k) plane_st(po, cap, CX + k, boff, nx[k]);
@@ -441,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 3))) v
                 sp.m = m;
                 sp.on = true;
                 sp.esc = !a.use_lds;
-                if (a.use_lds && stencil_escapes(a, sc, ch, slot, sp.b)) sp.esc = true;
+                if (a.use_lds && stencil_escapes(sc, ch, skey, sp.b)) sp.esc = true;
             }
         }
         // the earlier particles wait in LDS; the last one scatters them all

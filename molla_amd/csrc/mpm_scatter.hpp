@@ -715,13 +715,12 @@ __device__ __forceinline__ void block_scatter(const StepArgs& a, const SceneDev&
 }
 
 // True when a stencil starting at cell `b` leaves the 3x3x3 tiles around the
-// particle's sorted tile, i.e. the chunk partials of the last sort cannot
-// hold its contribution.  The sorted chunk's tile row (ty, tz) comes with the
-// chunk, so the particle's tile x is its key minus the row's first key (no
-// division); a chunk without a row tag takes the division.
-__device__ __forceinline__ bool stencil_escapes(const StepArgs& a, const SceneDev& sc, const Chunk& ch,
-                                                uint32_t slot, const int* b) {
-    const uint32_t key = a.sorted_keys[slot];
+// particle's sorted tile (tile key `key` at its slot), i.e. the chunk
+// partials of the last sort cannot hold its contribution.  The sorted chunk's
+// tile row (ty, tz) comes with the chunk, so the particle's tile x is its key
+// minus the row's first key (no division); a chunk without a row tag takes
+// the division.
+__device__ __forceinline__ bool stencil_escapes(const SceneDev& sc, const Chunk& ch, uint32_t key, const int* b) {
     int tk[3];
     if (ch.pad & 0x80000000u) {
         tk[1] = int(ch.pad & 0x7fffu);
