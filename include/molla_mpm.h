@@ -204,15 +204,15 @@ int mpm_set_launch(mpm_sim* sim, uint32_t particle_blocks, uint32_t grid_blocks)
  *                                 pass takes per chunk of the sorted layout
  *                                 (a chunk holds 256 x this many; shader
  *                                 model only, MPM_MODEL_CPUDIFF handles run 1);
- *   segment_tiles                 the chunk rule of the sorted layout; both
+ *   segment_tiles                 the chunk rule of the sorted layout; all
  *                                 keep a chunk inside one tile row, which
- *                                 bounds the LDS box of its scatter.  4
- *                                 (default): greedy rows, a chunk runs for
+ *                                 bounds the LDS box of its scatter.  4 or
+ *                                 6 (default): greedy rows, a chunk runs for
  *                                 256 x particles_per_thread particles or
- *                                 until it would span more than four tiles;
- *                                 8: aligned groups of eight tiles where
- *                                 they hold two chunks' worth, else of four,
- *                                 cut every 256 x particles_per_thread.
+ *                                 until it would span more than that many
+ *                                 tiles; 8: aligned groups of eight tiles
+ *                                 where they hold two chunks' worth, else of
+ *                                 four, cut every 256 x particles_per_thread.
  * The block is 256 threads (4 waves): with the 168-VGPR cap of three waves
  * per SIMD and three ~49 KB LDS boxes per CU, it is the size that fills a
  * CU.  Changing particles_per_thread or segment_tiles re-sorts at the next
@@ -239,7 +239,7 @@ int mpm_get_launch_config(mpm_sim* sim, mpm_launch_config* cfg);
  * from (stepping resumes from the same state).  Outputs may be NULL. */
 int mpm_autotune(mpm_sim* sim, float dt, uint32_t trial_substeps, uint32_t* particle_blocks,
                  uint32_t* grid_blocks, uint32_t* sort_every);
-/* The same search with particles_per_thread x segment_tiles first (all four
+/* The same search with particles_per_thread x segment_tiles first (all six
  * pairs, shader model), then the three above; `out` (may be NULL) receives
  * the configuration applied.  mpm_autotune runs this search too. */
 int mpm_autotune_config(mpm_sim* sim, float dt, uint32_t trial_substeps, mpm_launch_config* out);

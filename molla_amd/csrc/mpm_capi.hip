@@ -123,7 +123,7 @@ struct mpm_sim {
     uint32_t pblocks = 0;
     uint32_t gblocks = 0;
     uint32_t ppt = 2;         // particles per thread asked for (shader model)
-    uint32_t seg_tiles = 4;   // chunk rule: 4 greedy rows of at most four tiles, 8 aligned segments
+    uint32_t seg_tiles = 6;   // chunk rule: 4 or 6 greedy rows of at most that many tiles, 8 aligned segments
     uint32_t chunk_ppt = 1;   // particles per thread of the current sorted chunk list
     // profiling
     int profiling = 0;         // MPM_PROFILE_* flags
@@ -1394,7 +1394,7 @@ int autotune(mpm_sim* s, float dt, uint32_t trial, uint32_t* out_pb, uint32_t* o
     const uint32_t gb0 = best_gb, se0 = best_se ? best_se : 16u;
     if (s->sh[0].model == MPM_MODEL_SHADER && s->p2g_mode == MPM_P2G_FUSED) {
         for (uint32_t pp : {1u, 2u})
-            for (uint32_t sg : {4u, 8u}) {
+            for (uint32_t sg : {4u, 6u, 8u}) {
                 if (rc != MPM_OK) break;
                 try_ppt = pp;
                 try_seg = sg;
@@ -1701,8 +1701,8 @@ int mpm_set_launch_config(mpm_sim* sim, const mpm_launch_config* cfg) {
     if (!cfg) return fail(sim, MPM_ERR_INVALID, "cfg must not be NULL");
     if (cfg->particles_per_thread < 1 || cfg->particles_per_thread > uint32_t(mpm::MAX_PPT))
         return fail(sim, MPM_ERR_INVALID, "particles_per_thread must be 1 or 2");
-    if (cfg->segment_tiles != 4 && cfg->segment_tiles != 8)
-        return fail(sim, MPM_ERR_INVALID, "segment_tiles must be 4 or 8");
+    if (cfg->segment_tiles != 4 && cfg->segment_tiles != 6 && cfg->segment_tiles != 8)
+        return fail(sim, MPM_ERR_INVALID, "segment_tiles must be 4, 6 or 8");
     sim->pblocks = cfg->particle_blocks;
     sim->gblocks = cfg->grid_blocks;
     sim->sort_every = cfg->sort_every;

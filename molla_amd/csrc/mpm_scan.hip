@@ -125,7 +125,7 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 // Chunk list of the sorted particle array.  Two rules, both keeping a chunk
 // inside one tile row (so its scatter box is at most the row's 4 x 4 cross
 // section plus the stencil reach in y and z):
-//   greedy rows (span tiles, the default with span 4): a chunk starts at
+//   greedy rows (span 4 or 6 tiles; 6 is the default): a chunk starts at
 //     each row's first particle and runs for chunk_cap particles or until
 //     the next particle would make it span more than `span` tiles, so chunks
 //     start anywhere inside a tile and only a row's last chunk (or one cut by
@@ -133,8 +133,10 @@ hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint3
 //   aligned segments: chunks of chunk_cap from the start of each aligned
 //     group of four tiles, or of eight where those eight hold at least two
 //     chunks' worth.
-// The greedy rule makes ~10 % fewer chunks on the 2 M scene and ~14 % fewer
-// on the 512-scene batch, with fewer partly filled blocks.
+// Greedy rows of four tiles make ~10 % fewer chunks than the aligned rule on
+// the 2 M scene and ~14 % fewer on the 512-scene batch; six tiles ~11 % and
+// ~18 % (boxes up to 27 x 7 x 7 cells at the sort, inside the 1600-cell LDS
+// box with room for the motion until the next sort).
 //
 // The aligned segment of tile `key`: its first tile and its length along x.
 // lb is the tile lower-bound table of the sorted keys.
@@ -237,9 +239,9 @@ hipError_t build_sorted_chunks(const SceneDev* scenes, const uint32_t* tile_scen
                                uint32_t* scan_scratch, Chunk* chunks, uint32_t* n_chunks,
                                uint32_t* chunk_of_slot, uint32_t chunk_cap, uint32_t seg_tiles, uint32_t total_tiles,
                                hipStream_t s) {
-    if (chunk_cap == 0 || chunk_cap > CHUNK_PARTICLES * MAX_PPT || (seg_tiles != 4 && seg_tiles != 8))
+    if (chunk_cap == 0 || chunk_cap > CHUNK_PARTICLES * MAX_PPT || (seg_tiles != 4 && seg_tiles != 6 && seg_tiles != 8))
         return hipErrorInvalidValue;
-    const bool greedy = seg_tiles == 4;
+    const bool greedy = seg_tiles != 8;  // 4, 6: greedy rows of at most that many tiles
     const uint32_t dense_group = 2 * chunk_cap;
     const unsigned blocks = std::max(1u, std::min<unsigned>((n + 255) / 256, 8192u));
     if (greedy) {

@@ -421,8 +421,9 @@ class MpmSimulation:
 
     def set_launch_config(self, **kw):
         """Change any of particle_blocks, grid_blocks, sort_every,
-        particles_per_thread (1, 2) and segment_tiles (the chunk rule: 4 greedy
-        rows of at most four tiles, 8 aligned eight- or four-tile segments)."""
+        particles_per_thread (1, 2) and segment_tiles (the chunk rule: 4 or 6
+        greedy rows of at most that many tiles, 8 aligned eight- or four-tile
+        segments)."""
         cur = self.launch_config()
         unknown = set(kw) - set(cur)
         if unknown:

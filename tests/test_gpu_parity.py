@@ -461,11 +461,11 @@ def test_escaped_particles_fall_back_to_global_atomics():
     assert e <= max(8 * e32, 1e-6)
 
 
-@pytest.mark.parametrize("ppt,seg", [(1, 4), (1, 8), (2, 4), (2, 8)])
+@pytest.mark.parametrize("ppt,seg", [(1, 4), (1, 8), (2, 4), (2, 6), (2, 8)])
 def test_launch_configurations_match_fp64_oracle_and_repeat_bitwise(ppt, seg):
     # Particles per thread and the chunk rule change the chunks (256 or 512
-    # particles; greedy rows of at most 4 tiles, or aligned 4- and 8-tile
-    # segments) and with them each chunk's fixed-point scale, so results
+    # particles; greedy rows of at most 4 or 6 tiles, or aligned 4- and
+    # 8-tile segments) and with them each chunk's fixed-point scale, so results
     # differ across configurations at f32 rounding only; one configuration
     # run twice is bitwise identical.  The cube holds 16 particles per cell,
     # so 8-tile segments and full 512-particle chunks occur, and with 11
@@ -511,7 +511,7 @@ def test_launch_configuration_round_trips_and_rejects_bad_values():
     sim.set_launch_config(particle_blocks=64, grid_blocks=128, sort_every=9, particles_per_thread=1, segment_tiles=4)
     assert sim.launch_config() == {"particle_blocks": 64, "grid_blocks": 128, "sort_every": 9,
                                    "particles_per_thread": 1, "segment_tiles": 4}
-    for bad in ({"particles_per_thread": 0}, {"particles_per_thread": 3}, {"segment_tiles": 6}):
+    for bad in ({"particles_per_thread": 0}, {"particles_per_thread": 3}, {"segment_tiles": 5}):
         with pytest.raises(MpmError):
             sim.set_launch_config(**bad)
     assert sim.launch_config()["particles_per_thread"] == 1
@@ -534,23 +534,23 @@ def test_launch_configuration_round_trips_and_rejects_bad_values():
 
 def _model_chunk_count(tile_counts: np.ndarray, W: int, cap: int, segment_tiles: int) -> int:
     """Chunks the sorted layout's chunk rule makes for one scene whose tiles
-    (x fastest) hold tile_counts particles.  segment_tiles 4: greedy rows (a
-    chunk runs for cap particles or until it would span more than four
-    tiles); 8: aligned segments of eight tiles holding at least 2 cap
-    particles, else of four, cut every cap particles."""
+    (x fastest) hold tile_counts particles.  segment_tiles 4 or 6: greedy
+    rows (a chunk runs for cap particles or until it would span more than
+    that many tiles); 8: aligned segments of eight tiles holding at least
+    2 cap particles, else of four, cut every cap particles."""
     n = 0
     for row in range(W * W):
         c = tile_counts[row * W:(row + 1) * W]
         if not c.any():
             continue
         lb = np.concatenate([[0], np.cumsum(c)])
-        if segment_tiles == 4:
+        if segment_tiles in (4, 6):
             s, tt = 0, 0
             while s < lb[W]:
                 n += 1
                 while lb[tt + 1] <= s:
                     tt += 1
-                s = min(s + cap, lb[min(tt + 4, W)])
+                s = min(s + cap, lb[min(tt + segment_tiles, W)])
         else:
             for g8 in range(0, W, 8):
                 n8 = min(8, W - g8)
@@ -572,7 +572,7 @@ def test_chunk_rules_make_the_modelled_chunk_count(ppt):
     cube_cfg, cube_x, cube_meta = workloads.elastic_cube(8192, seed=3)
     for case, (cs, ps, dt) in {"cube": ([cube_cfg], [cube_x], cube_meta["dt"]),
                                "batch": (cfgs, xs, meta["dt"])}.items():
-        for seg in (4, 8):
+        for seg in (4, 6, 8):
             want = 0
             for c, x in zip(cs, ps):
                 W = (c.grid_size + 3) // 4
@@ -599,7 +599,7 @@ def test_autotune_restores_state_and_keeps_results():
     x0, F0 = sim.read_positions(), sim.read_field("F")
     tuned = sim.autotune(dt, 4)
     log(f"autotune -> {tuned}")
-    assert tuned["particles_per_thread"] in (1, 2) and tuned["segment_tiles"] in (4, 8)
+    assert tuned["particles_per_thread"] in (1, 2) and tuned["segment_tiles"] in (4, 6, 8)
     assert sim.launch_config() == tuned
     assert np.array_equal(sim.read_positions(), x0) and np.array_equal(sim.read_field("F"), F0)
     st = sim.read_status()
