@@ -478,14 +478,15 @@ __global__ __launch_bounds__(256) void k_adj_grid(const SceneDev* __restrict__ s
     const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
     const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
     for (uint32_t i = first; i < ns + nd; i += stride) {
-        const int* rec = g.srec && i < ns ? g.srec + (size_t)i * SREC_WORDS : nullptr;
-        const uint32_t t = rec ? uint32_t(rec[0]) : (i < ns ? g.slist[i] : dlist[i - ns]);
-        const SceneDev sc = scenes[rec ? uint32_t(rec[1]) : g.tile_scene[t]];
+        TileRec r;
+        load_tile_rec(g, ns, i, r);
+        const uint32_t t = r.valid ? r.t : (i < ns ? g.slist[i] : dlist[i - ns]);
+        const SceneDev sc = scenes[r.valid ? r.scene : g.tile_scene[t]];
         int tx, ty, tz, ix, iy, iz;
         tile_cell(sc, t, lane, tx, ty, tz, ix, iy, iz);
         const uint32_t c = (t << 6) | uint32_t(lane);
         float4 q = use_esc && (!SPLIT || wave == 0) ? gadj[c] : make_float4(0.f, 0.f, 0.f, 0.f);
-        add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? R : 1, rec);
+        add_partials(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0, SPLIT ? R : 1, r.valid, r.rng);
         if (SPLIT) {
             if (wave > 0) wsum[wave - 1][lane] = q;
             __syncthreads();

@@ -570,14 +570,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 :
     const int lane = threadIdx.x & 63, wave = wave_id();
     const uint32_t first = SPLIT ? blockIdx.x : blockIdx.x * 4 + uint32_t(wave);
     const uint32_t stride = SPLIT ? gridDim.x : gridDim.x * 4;
+    // A static tile's record holds its id, scene and candidate ranges; the
+    // next tile's record is requested before this tile's stores (which the
+    // compiler cannot move it past), so a wave that takes several tiles
+    // finds it loaded.
+    TileRec cur, nxt;
+    if (first < walk.n) load_tile_rec(g, walk.ns, first, cur);
     for (uint32_t i = first; i < walk.n; i += stride) {
+        nxt.valid = false;
+        if (i + stride < walk.n) load_tile_rec(g, walk.ns, i + stride, nxt);
         uint32_t t, st, scene;
-        // a static tile's record holds its id, scene and candidate ranges
-        const int* rec = g.srec && i < walk.ns ? g.srec + (size_t)i * SREC_WORDS : nullptr;
-        if (rec) {
-            t = uint32_t(rec[0]);
+        if (cur.valid) {
+            t = cur.t;
             st = t;
-            scene = uint32_t(rec[1]);
+            scene = cur.scene;
         } else {
             walk.at(g, dlist, i, t, st);
             scene = g.tile_scene[t];
@@ -591,7 +597,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 :
         float4 q = (flag & FLAG_GMP) && (!SPLIT || wave == 0) ? g.gmp[c] : make_float4(0.f, 0.f, 0.f, 0.f);
         if (use_partials) {
             add_partials<(SPLIT || RIGID) ? 8 : 4>(g, sc, tx, ty, tz, ix, iy, iz, q, SPLIT ? wave : 0,
-                                                   SPLIT ? GRID_RANGES : 1, rec);
+                                                   SPLIT ? GRID_RANGES : 1, cur.valid, cur.rng);
             if (SPLIT) {
                 if (wave > 0) wsum[wave - 1][lane] = q;
                 __syncthreads();
@@ -633,6 +639,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RIGID ? 4 :
             }
         }
         if (SPLIT && use_partials) __syncthreads();  // wsum is reused by the next tile
+        cur = nxt;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && next_count) *next_count = 0u;
 }

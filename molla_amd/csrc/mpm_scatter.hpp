@@ -803,23 +803,22 @@ __device__ __forceinline__ bool stage_gather_box(const StepArgs& a, const SceneD
 // part / nparts: this wave adds only the part-th of nparts contiguous ranges
 // of the candidate list, in ascending order, starting from q (the waves of a
 // block can share a tile's candidates; the caller adds their sums in part
-// order).  nparts = 1: the whole list.  rec (may be null): the tile's static
-// record (GridDev::srec), which holds the nine rows' candidate ranges.
+// order).  nparts = 1: the whole list.  use_rng: rng holds the nine rows'
+// candidate ranges from the tile's static record (GridDev::srec), lane r < 9
+// holding row r's (first, last), read by the caller.
 // U: overlapping chunks whose partial loads are in flight together.
 template <int U = 8>
 __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& sc, int tx, int ty, int tz,
                                              int ix, int iy, int iz, float4& q, int part = 0, int nparts = 1,
-                                             const int* rec = nullptr) {
+                                             bool use_rng = false, int2 rng = int2{0, -1}) {
     const int lane = threadIdx.x & 63;
     const int W = int(sc.W);
     const int X0 = tx * 4, Y0 = ty * 4, Z0 = tz * 4;
     const int xa = max(tx - 1, 0), xb = min(tx + 1, W - 1);
     int rc0 = 0, rc1 = -1;  // empty row
-    if (rec) {
-        if (lane < 9) {
-            rc0 = rec[2 + 2 * lane];
-            rc1 = rec[3 + 2 * lane];
-        }
+    if (use_rng) {
+        rc0 = rng.x;
+        rc1 = rng.y;
     } else if (lane < 9) {
         const int nz = tz + lane / 3 - 1, ny = ty + lane % 3 - 1;
         if (nz >= 0 && nz < W && ny >= 0 && ny < W) {
@@ -896,6 +895,27 @@ __device__ __forceinline__ void add_partials(const GridDev& g, const SceneDev& s
                 }
             }
         }
+    }
+}
+
+// A static tile's record as the grid kernels use it: tile, scene and (lanes
+// 0-8) the nine rows' candidate ranges; valid = false for entries without one.
+struct TileRec {
+    uint32_t t, scene;
+    int2 rng;
+    bool valid;
+};
+__device__ __forceinline__ void load_tile_rec(const GridDev& g, uint32_t ns, uint32_t i, TileRec& r) {
+    r.valid = g.srec && i < ns;
+    r.t = 0u;
+    r.scene = 0u;
+    r.rng = make_int2(0, -1);
+    if (r.valid) {
+        const int* rec = g.srec + (size_t)i * SREC_WORDS;
+        r.t = uint32_t(rec[0]);
+        r.scene = uint32_t(rec[1]);
+        const int lane = threadIdx.x & 63;
+        if (lane < 9) r.rng = make_int2(rec[2 + 2 * lane], rec[3 + 2 * lane]);
     }
 }
 
