@@ -437,6 +437,39 @@ def test_sparse_scene_with_more_chunks_than_partial_slots_matches_oracle():
         assert e <= max(8 * e32, floor), (f, e, e32)
 
 
+def test_chunk_boxes_beyond_lds_fall_back_to_global_atomics():
+    # Lines of particles along x, one per tile row: greedy chunks span six
+    # tiles (27 cells with the stencil), and particles drifting in y and z
+    # within their sort neighbourhood grow a chunk's box past the 1600-cell
+    # LDS box before the next sort, so whole chunks scatter with global
+    # atomics (status.fallback_chunks).  Results must still match the oracle.
+    rng = np.random.default_rng(29)
+    dx = 1.0 / 64
+    rows = [(ty, tz) for ty in (5, 7, 9) for tz in (6, 9)]
+    xs = np.arange(0.2, 0.8, dx / 2)
+    x = np.array([[xx, (4 * ty + 2) * dx, (4 * tz + 2) * dx] for ty, tz in rows for xx in xs], np.float32)
+    v = np.zeros_like(x)
+    v[:, 1:] = rng.uniform(-3.0, 3.0, (len(x), 2)).astype(np.float32)
+    cfg = MpmConfig.sand().copy(gravity_y=0.0, rheology=Viscous(0, 0), youngs=1.0e3, grid_size=64, cell_size=dx,
+                                grid_origin=(0.0, 0.0, 0.0), max_particles=len(x))
+    sim = MpmSimulation(cfg, max_particles=len(x))
+    sim.set_particles(x, v)
+    sim.set_launch_config(particles_per_thread=2, segment_tiles=6, sort_every=40)
+    sim.step_n(1e-3, 40)
+    st = sim.read_status()
+    xg = sim.read_positions()
+    sim.close()
+    log(f"lines: {len(x)} particles, status {st}")
+    assert st["fallback_chunks"] > 0 and st["fault_bits"] == 0
+    o64 = oracle_state(cfg, x, v)
+    o32 = oracle_state(cfg, x, v, dtype=np.float32)
+    orc.step(cfg, o64, 1e-3, 40)
+    orc.step(cfg, o32, 1e-3, 40)
+    e, e32 = rel_err(xg, o64.x), rel_err(o32.x, o64.x)
+    log(f"  x err {e:.2e} (f32 oracle {e32:.2e})")
+    assert e <= max(8 * e32, 1e-6)
+
+
 def test_escaped_particles_fall_back_to_global_atomics():
     # Fast particles and a long sort interval: stencils leave their sorted
     # 3x3x3 tile neighbourhood, so those particle-substeps take the global
